@@ -1,0 +1,143 @@
+/*
+ * spinsim_hip.h -- C ABI of libspinsim_hip.so, the MI355X (gfx950) integration
+ * path of spinsim_amd.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns 0 on success or a nonzero status; the message of the last failure
+ * on the calling thread is returned by ss_last_error().  Device pointers are
+ * caller-allocated (any HIP allocation, e.g. a torch tensor's data_ptr());
+ * the library never frees caller memory.  Work is enqueued on `stream`
+ * (a hipStream_t; NULL is the null stream) and is ordered with other work on
+ * that stream; calls do not synchronise unless stated.  A module is immutable
+ * after loading and may be used from several host threads on distinct
+ * streams.
+ *
+ * Complex arrays are interleaved (re, im) float64 pairs -- the layout of
+ * numpy/torch complex128 -- with matrices row-major [row][col].
+ *
+ * Reference interfaces replaced (spinsim/__init__.py of the reference):
+ *   ss_compile / ss_compile_code / ss_module_load
+ *       Simulator.compile_time_evolver, :453-991 (numba specialisation of
+ *       the integrator on spin, method, exponentiation and frame, with the
+ *       user's get_field JIT-compiled as a device function, :620)
+ *   ss_time_evolution          get_time_evolution kernel, :854-896
+ *   ss_state_scan              Simulator.get_state, :1087-1111
+ *   ss_spin                    get_spin / spin_calculator, :898-988
+ *   ss_batch_time_evolution    evaluate() called once per sweep, :993-1085
+ *   ss_batch_state_scan        get_state once per simulation, :1087-1111
+ *   ss_interp_eval             the host-callable sampler of
+ *                              generate_interpolation_sampler, :1941-2142
+ */
+#ifndef SPINSIM_HIP_H
+#define SPINSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ss_module ss_module;
+typedef void *ss_stream; /* hipStream_t */
+
+/* One interpolation table of generate_interpolation_sampler
+ * (spinsim/__init__.py:1941-2142), in device memory.  amplitude_stride > 0
+ * selects row `sim` of a (B, size) amplitude array in batched launches. */
+typedef struct ss_table {
+    const double *time;
+    const double *amplitude;
+    int64_t size;
+    int64_t amplitude_stride;
+    int32_t order;         /* 0 zero-order hold, 1 linear, 3 cubic Hermite */
+    int32_t zero_boundary; /* 1: zero outside [time[0], time[-1]] */
+} ss_table;
+
+enum { /* IntegrationMethod, :68-93 */
+    SS_METHOD_MAGNUS_CF4 = 0,
+    SS_METHOD_MIDPOINT_SAMPLE = 1, /* the reference's EULER */
+    SS_METHOD_HALF_STEP = 2        /* the reference's HEUN */
+};
+enum { /* ExponentiationMethod, :95-129 (LIE_TROTTER_8 not provided) */
+    SS_EXPO_ANALYTIC = 0,
+    SS_EXPO_LIE_TROTTER = 1
+};
+
+/* Message of the last failure on this thread ("" if none). */
+const char *ss_last_error(void);
+/* ABI version of this header (bumped on incompatible change). */
+int ss_abi_version(void);
+/* 64-bit hash of the embedded kernel sources (for code-object caches). */
+uint64_t ss_source_hash(void);
+
+/* Compile a field callback with the integrator for gfx950 via hipRTC; no GPU
+ * needed.  `field_src` defines
+ *   __device__ void get_field(double t, const double *sweep, double *field)
+ * or, with field_has_context = 1, the same with a trailing SS_CONTEXT
+ * argument (for samplers).  dim is 2 (spin half) or 3 (spin one);
+ * number_of_squares as in the reference (Lie-Trotter squarings, :1547-1550).
+ * On success *code / *code_size hold a code object and *log the compiler log
+ * (both freed with ss_free); on failure *log holds the log when available. */
+int ss_compile_code(const char *field_src, int dim, int method, int expo, int rotating,
+                    int number_of_squares, int field_has_context, void **code,
+                    size_t *code_size, char **log);
+void ss_free(void *p);
+
+/* Load a code object from ss_compile_code on the current HIP device. */
+int ss_module_load(const void *code, size_t code_size, int dim, int method, int expo,
+                   int rotating, ss_module **out);
+/* ss_compile_code + ss_module_load. */
+int ss_compile(const char *field_src, int dim, int method, int expo, int rotating,
+               int number_of_squares, int field_has_context, ss_module **out);
+int ss_module_free(ss_module *m);
+/* Resource use of the single-simulation kernel (any pointer may be NULL). */
+int ss_module_info(const ss_module *m, int *vgprs, int *sgprs, int *scratch_bytes,
+                   int *max_threads);
+
+/* Time-evolution operator of each coarse step of one simulation:
+ * time_out[i] = t0 + dt_out*i and u_out[i] = the product of the
+ * floor(dt_out/dt + 0.5) step exponentials of [time_out[i], time_out[i] +
+ * dt_out), taken out of the rotating frame.  `dt` is the integration step as
+ * evaluate() re-derives it.  sweep: device array read by get_field; tables:
+ * device ss_table array or NULL.  lanes_per_step in 1..64 lanes share one
+ * coarse step (0 = library default).  u_out: n_coarse*dim*dim complex128. */
+int ss_time_evolution(const ss_module *m, const double *sweep, double t0, double dt,
+                      double dt_out, int64_t n_coarse, double *time_out, double *u_out,
+                      const ss_table *tables, int lanes_per_step, ss_stream stream);
+
+/* Batched time evolution: simulation s owns rows [offsets[s], offsets[s+1])
+ * of time_out / u_out and reads sweep row s (sweep + s*sweep_stride).  All
+ * arrays are device arrays; offsets has n_sims+1 entries, offsets[n_sims] =
+ * total.  counter: one device uint64 the library clears before the launch.
+ * grid_blocks = 0 picks a persistent grid for the device. */
+int ss_batch_time_evolution(const ss_module *m, const double *sweep, int64_t sweep_stride,
+                            int64_t n_sims, const int64_t *offsets, const double *t0,
+                            const double *dt, const double *dt_out, const int32_t *n_fine,
+                            int64_t total, double *time_out, double *u_out,
+                            const ss_table *tables, uint64_t *counter, int lanes_per_step,
+                            int items_per_grab, int grid_blocks, ss_stream stream);
+
+/* Workspace bytes ss_state_scan needs for n rows of dimension dim. */
+size_t ss_state_scan_workspace(int64_t n, int dim);
+/* state[0] = psi0, state[i] = u[i-1] state[i-1] for i < n (u[n-1] unused),
+ * by a parallel prefix product in float64.  psi0: HOST array of dim complex
+ * values; u, state, workspace: device. */
+int ss_state_scan(const double *psi0, const double *u, double *state, int64_t n, int dim,
+                  void *workspace, size_t workspace_bytes, ss_stream stream);
+/* Per-simulation state scan over CSR rows; psi0: device (n_sims, dim). */
+int ss_batch_state_scan(const double *psi0, const double *u, double *state,
+                        const int64_t *offsets, int64_t n_sims, int dim, ss_stream stream);
+
+/* Evaluate interpolation table k (row `sim` when batched) at n device
+ * times: the sampler of generate_interpolation_sampler, :2065-2142. */
+int ss_interp_eval(const ss_table *tables, int k, int sim, const double *t, double *out, int64_t n,
+                   ss_stream stream);
+
+/* Bloch vector of each of n states (dim complex each): spin is (n, 3). */
+int ss_spin(const double *state, double *spin, int64_t n, int dim, ss_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPINSIM_HIP_H */

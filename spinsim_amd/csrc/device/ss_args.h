@@ -1,0 +1,46 @@
+// Kernel argument blocks shared by the host library and the hipRTC-compiled
+// integrator.  Plain C layout, 8-byte aligned, passed by value.
+#pragma once
+
+// Single-simulation time-evolution launch: one coarse (output) step per group
+// of `lanes_per_step` lanes, mirroring the reference kernel
+// get_time_evolution(sweep, time_coarse, time_end_points, dt, dt_out, U)
+// (spinsim/__init__.py:854-896).
+struct ss_te_args {
+    const double* sweep;         // sweep_parameters (device)
+    const void* tables;          // ss_table[] (device) or null
+    double* time_out;            // (N,) start time of each coarse step
+    double* u_out;               // (N, d, d) complex128, row-major
+    long long n_coarse;          // N
+    double t0;                   // time_end_points[0]
+    double dt;                   // integration step (already re-derived)
+    double dt_out;               // output step
+    double off_a;                // first sample offset from the fine-step start
+    double off_b;                // second sample offset (CF4 / HEUN)
+    int n_fine;                  // floor(dt_out/dt + 0.5)
+    int lanes_per_step;          // lanes sharing one coarse step (1..64)
+};
+
+// Batched launch: simulation s owns coarse rows [offsets[s], offsets[s+1]).
+// Each simulation has its own sweep row, start time and steps.  Work is
+// handed out in units of `items_per_grab` coarse rows from a global counter,
+// so long and short simulations share the machine evenly.
+struct ss_batch_args {
+    const double* sweep;         // (B, sweep_stride)
+    const void* tables;          // ss_table[] or null
+    const long long* offsets;    // (B + 1,) CSR offsets into the coarse rows
+    const double* t0;            // (B,)
+    const double* dt;            // (B,) integration step (re-derived)
+    const double* dt_out;        // (B,)
+    const int* n_fine;           // (B,)
+    double* time_out;            // (total,)
+    double* u_out;               // (total, d, d) complex128
+    unsigned long long* counter; // work counter, zero before launch
+    long long total;             // offsets[B]
+    long long n_sims;            // B
+    long long sweep_stride;      // doubles per sweep row
+    double off_a_frac;           // sample offsets as multiples of 0.5*dt
+    double off_b_frac;
+    int lanes_per_step;
+    int items_per_grab;
+};

@@ -1,0 +1,435 @@
+// spinsim_amd time-evolution kernels (compiled by hipRTC after the field
+// callback, specialised by these macros):
+//   SS_DIM 2|3          spin-half / spin-one
+//   SS_LIE 3|4          field entries used (x, y, z[, q])
+//   SS_METHOD 0|1|2     magnus_cf4 / midpoint sample (euler) / half step (heun)
+//   SS_EXPO 0|1         analytic (spin-half) / Lie-Trotter
+//   SS_ROT 0|1          rotating frame
+//   SS_HYPER h          Lie-Trotter hypercubes (2h residual squarings)
+//   SS_CTX 0|1          callback takes the sampler context argument
+//
+// Each coarse (output) step i integrates n_fine sub-steps into U[i]
+// (spinsim/__init__.py:796-852).  A group of G lanes shares one coarse step:
+// lane r integrates a contiguous chunk of the sub-steps, and the G partial
+// products are combined in time order with a shuffle tree, so short output
+// intervals still give the chip enough independent lanes.
+#pragma once
+
+#define SS_D SS_DIM
+#define SS_FIELD_LEN 8
+
+#define SS_W0 0.5386751345948129       // (1.5 + sqrt3)/6, spinsim/__init__.py:659
+#define SS_W1 (-0.038675134594812866)  // (1.5 - sqrt3)/6, :660
+#define SS_INV_SQRT2 0.7071067811865475
+#define SS_INV_SIX 0.16666666666666666
+#define SS_INV_P (1.0 / (double)(1ull << (2 * SS_HYPER)))  // 1 / 4**hypercubes
+
+// sin and cos of x.  The Lie-Trotter generators are scaled by 4^-hypercubes,
+// so their angles are tiny; there the Taylor series to x^9 / x^10 is exact to
+// double precision (|x| < 1/16: truncation below 3e-20 relative).  Larger
+// arguments go to the device libm.
+static __device__ __forceinline__ void ss_sincos(double x, double* s, double* c)
+{
+    if (fabs(x) < 0.0625) {
+        const double x2 = x * x;
+        *s = fma(x * x2,
+                 fma(x2, fma(x2, fma(x2, 2.7557319223985893e-06, -0.0001984126984126984),
+                             0.008333333333333333),
+                     -0.16666666666666666),
+                 x);
+        *c = fma(x2,
+                 fma(x2,
+                     fma(x2, fma(x2, fma(x2, -2.755731922398589e-07, 2.48015873015873e-05),
+                                 -0.001388888888888889),
+                         0.041666666666666664),
+                     -0.5),
+                 1.0);
+    } else {
+        sincos(x, s, c);
+    }
+}
+
+// Residual square: for A = 1 + a, a <- A^2 - 1 = a (a + 2)
+// (matrix_square_m1, spinsim/__init__.py:1589-1593 / :1718-1727).
+static __device__ __forceinline__ void ss_square_m1(ss_cd a[SS_D][SS_D])
+{
+    ss_cd b[SS_D][SS_D];
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) b[r][c] = (r == c) ? ss_c(a[r][c].x + 2.0, a[r][c].y) : a[r][c];
+    ss_cd t[SS_D][SS_D];
+    ss_matmul<SS_D>(a, b, t);
+    ss_copy<SS_D>(t, a);
+}
+
+// ---- matrix exponentials of -i g.J ----------------------------------------
+#if SS_EXPO == 0 && SS_DIM == 2
+// matrix_exponential_analytic, spinsim/__init__.py:1617-1641.
+// cos_exp_m1(r/2, 0) = -2 sin^2(r/4); sin(r/2) from the same sincos.
+static __device__ __forceinline__ void ss_expo(const double g[SS_FIELD_LEN], ss_cd e[2][2])
+{
+    double x = g[0], y = g[1], z = g[2];
+    const double r = sqrt(x * x + y * y + z * z);
+    if (r > 0) {
+        const double ir = 1.0 / r;
+        x *= ir;
+        y *= ir;
+        z *= ir;
+        double s4, c4;
+        ss_sincos(0.25 * r, &s4, &c4);
+        const double c = -2.0 * s4 * s4;
+        const double s = 2.0 * s4 * c4;
+        e[0][0] = ss_c(c + 1.0, -z * s);
+        e[1][0] = ss_c(y * s, -x * s);
+        e[0][1] = ss_c(-y * s, -x * s);
+        e[1][1] = ss_c(c + 1.0, z * s);
+    } else {
+        ss_set_identity<SS_D>(e);
+    }
+}
+#elif SS_DIM == 2
+// matrix_exponential_lie_trotter (spin half), spinsim/__init__.py:1643-1673.
+static __device__ __forceinline__ void ss_expo(const double g[SS_FIELD_LEN], ss_cd e[2][2])
+{
+    double a = sqrt(g[0] * g[0] + g[1] * g[1]);
+    ss_cd ep = ss_c(1.0, 0.0);
+    if (a > 0) {
+        const double ia = 1.0 / a;
+        ep = ss_c(g[0] * ia, g[1] * ia);
+    }
+    a *= SS_INV_P;
+    const double z = g[2] * (0.5 * SS_INV_P);
+    double s4, c4, sz, cz;
+    ss_sincos(0.25 * a, &s4, &c4);
+    ss_sincos(0.5 * z, &sz, &cz);
+    const double cosm1_h = -2.0 * s4 * s4;           // cos(a/2) - 1
+    const double sin_h = 2.0 * s4 * c4;              // sin(a/2)
+    const ss_cd mz = ss_c(-2.0 * sz * sz, 2.0 * sz * cz);  // exp(iz) - 1
+    const ss_cd ez = ss_c(1.0 + mz.x, mz.y);
+    const ss_cd sa = ss_c(0.0, -sin_h);              // -i sin(a/2)
+    // cos_exp_m1(a/2, -+z) = (cos(a/2) - 1) exp(-+iz) + expm1i(-+z)
+    e[0][0] = ss_add(ss_scale(ss_conj(ez), cosm1_h), ss_conj(mz));
+    e[1][1] = ss_add(ss_scale(ez, cosm1_h), mz);
+    e[1][0] = ss_mul(sa, ep);
+    e[0][1] = ss_mul(sa, ss_conj(ep));
+#pragma unroll 1
+    for (int h = 0; h < SS_HYPER; ++h) {
+        ss_square_m1(e);
+        ss_square_m1(e);
+    }
+    e[0][0].x += 1.0;
+    e[1][1].x += 1.0;
+}
+#else
+// matrix_exponential_lie_trotter (spin one), spinsim/__init__.py:1767-1807.
+// The reference forms each entry from atan2 and complex exponentials of
+// (q +- p +- z); here exp(ip) = (g_x + i g_y)/|g_xy| and the phases are
+// products of exp(i(z+q)), exp(i(z-q)) and exp(2iq), whose residuals
+// exp(ib) - 1 come from half-angle sines without cancellation.
+static __device__ __forceinline__ void ss_expo(const double g[SS_FIELD_LEN], ss_cd e[3][3])
+{
+    double a = sqrt(g[0] * g[0] + g[1] * g[1]);
+    ss_cd ep = ss_c(1.0, 0.0);
+    if (a > 0) {
+        const double ia = 1.0 / a;
+        ep = ss_c(g[0] * ia, g[1] * ia);
+    }
+    a *= SS_INV_P;
+    const double z = g[2] * (0.5 * SS_INV_P);
+    const double q = (g[3] * SS_INV_SIX) * SS_INV_P;
+    double s4, c4, s1, c1, s2, c2, s3, c3;
+    ss_sincos(0.25 * a, &s4, &c4);
+    ss_sincos(0.5 * (z + q), &s1, &c1);
+    ss_sincos(0.5 * (z - q), &s2, &c2);
+    ss_sincos(2.0 * q, &s3, &c3);
+    const double sin_h = 2.0 * s4 * c4;          // sin(a/2)
+    const double cosm1_h = -2.0 * s4 * s4;       // cos(a/2) - 1
+    const double sin_a = 2.0 * sin_h * (1.0 + cosm1_h);
+    const double cosm1_a = -2.0 * sin_h * sin_h; // cos(a) - 1
+    const ss_cd sa = ss_c(0.0, -sin_a * SS_INV_SQRT2);
+    const ss_cd m1 = ss_c(-2.0 * s1 * s1, 2.0 * s1 * c1);   // exp(i(z+q)) - 1
+    const ss_cd m2 = ss_c(-2.0 * s2 * s2, 2.0 * s2 * c2);   // exp(i(z-q)) - 1
+    const ss_cd m4 = ss_c(-2.0 * s3 * s3, 2.0 * s3 * c3);   // exp(4iq) - 1
+    const ss_cd e1 = ss_c(1.0 + m1.x, m1.y);
+    const ss_cd e2 = ss_c(1.0 + m2.x, m2.y);
+    const ss_cd e4 = ss_c(1.0 + m4.x, m4.y);
+    // diagonal residuals: save*(save + 2) with save = cos_exp_m1(a/2, .)
+    const ss_cd save0 = ss_add(ss_scale(ss_conj(e1), cosm1_h), ss_conj(m1));  // e = -z-q
+    const ss_cd save2 = ss_add(ss_scale(e2, cosm1_h), m2);                    // e = z-q
+    e[0][0] = ss_mul(save0, ss_c(save0.x + 2.0, save0.y));
+    e[2][2] = ss_mul(save2, ss_c(save2.x + 2.0, save2.y));
+    e[1][1] = ss_add(ss_scale(e4, cosm1_a), m4);                             // cos_exp_m1(a, 4q)
+    const ss_cd sep = ss_mul(sa, ep);
+    const ss_cd sem = ss_mul(sa, ss_conj(ep));
+    e[1][0] = ss_mul(sep, ss_conj(e2));   // sa exp(i(q + p - z))
+    e[0][1] = ss_mul(sem, ss_conj(e2));   // sa exp(i(q - p - z))
+    e[2][1] = ss_mul(sep, e1);            // sa exp(i(q + p + z))
+    e[1][2] = ss_mul(sem, e1);            // sa exp(i(q - p + z))
+    const ss_cd ep2 = ss_mul(ep, ep);
+    const ss_cd w = ss_mul(ep2, ss_c(c3, -s3));  // exp(2i(p - q))
+    const double sh2 = -sin_h * sin_h;
+    e[2][0] = ss_scale(w, sh2);
+    e[0][2] = ss_scale(ss_conj(w), sh2);
+#pragma unroll 1
+    for (int h = 0; h < SS_HYPER; ++h) {
+        ss_square_m1(e);
+        ss_square_m1(e);
+    }
+    e[0][0].x += 1.0;
+    e[1][1].x += 1.0;
+    e[2][2].x += 1.0;
+}
+#endif
+
+// ---- field sampling and frame ----------------------------------------------
+static __device__ __forceinline__ void ss_call_field(double t, const double* sweep, double f[SS_FIELD_LEN],
+                                                     const ss_ctx* ctx)
+{
+#pragma unroll
+    for (int k = 0; k < SS_FIELD_LEN; ++k) f[k] = 0.0;
+#if SS_CTX
+    get_field(t, sweep, f, ctx);
+#else
+    (void)ctx;
+    get_field(t, sweep, f);
+#endif
+}
+
+// transform_frame_spin_{half,one}_rotating, spinsim/__init__.py:599-613.
+static __device__ __forceinline__ void ss_frame(double f[SS_FIELD_LEN], double rot, ss_cd w)
+{
+#if SS_ROT
+#if SS_DIM == 2
+    const ss_cd wc = ss_conj(ss_mul(w, w));
+    const ss_cd x = ss_mul(ss_c(f[0], f[1]), wc);
+    f[2] = f[2] - 2.0 * rot;
+#else
+    const ss_cd x = ss_mul(ss_c(f[0], f[1]), ss_conj(w));
+    f[2] = f[2] - rot;
+#endif
+    f[0] = x.x;
+    f[1] = x.y;
+#else
+    (void)f;
+    (void)rot;
+    (void)w;
+#endif
+}
+
+static __device__ __forceinline__ ss_cd ss_winding(double rot, double t_rel)
+{
+#if SS_ROT
+    double s, c;
+    sincos(rot * t_rel, &s, &c);
+    return ss_c(c, s);
+#else
+    (void)rot;
+    (void)t_rel;
+    return ss_c(1.0, 0.0);
+#endif
+}
+
+static __device__ __forceinline__ void ss_append(const double g[SS_FIELD_LEN], ss_cd u[SS_D][SS_D])
+{
+    ss_cd e[SS_D][SS_D];
+    ss_expo(g, e);
+    ss_left_multiply<SS_D>(e, u);
+}
+
+// `count` fine steps starting at fine index j0 of the coarse step starting at
+// tc; u (initially identity) accumulates them.  The fine time is advanced by
+// repeated addition from tc exactly like the reference (:819, :836).
+static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0, int count, double tc, double rot,
+                                                    double dt, double off_a, double off_b,
+                                                    const double* sweep, const ss_ctx* ctx)
+{
+    double tf = tc;
+#pragma unroll 1
+    for (int j = 0; j < j0; ++j) tf += dt;
+    ss_set_identity<SS_D>(u);
+#pragma unroll 1
+    for (int j = 0; j < count; ++j) {
+        double fa[SS_FIELD_LEN], g[SS_FIELD_LEN];
+#if SS_METHOD == 1
+        // euler (midpoint sample), :758-791
+        const double tr = (tf + off_a) - tc;
+        const ss_cd wa = ss_winding(rot, tr);
+        ss_call_field(tr + tc, sweep, fa, ctx);
+        ss_frame(fa, rot, wa);
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fa[k];
+        ss_append(g, u);
+#else
+        double fb[SS_FIELD_LEN];
+        double tr = (tf + off_a) - tc;
+        const ss_cd wa = ss_winding(rot, tr);
+        ss_call_field(tr + tc, sweep, fa, ctx);
+        tr = (tf + off_b) - tc;
+        const ss_cd wb = ss_winding(rot, tr);
+        ss_call_field(tr + tc, sweep, fb, ctx);
+        ss_frame(fa, rot, wa);
+        ss_frame(fb, rot, wb);
+#if SS_METHOD == 0
+        // magnus_cf4, :655-686
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * (SS_W0 * fa[k] + SS_W1 * fb[k]);
+        ss_append(g, u);
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * (SS_W1 * fa[k] + SS_W0 * fb[k]);
+        ss_append(g, u);
+#else
+        // heun (half step), :724-752
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fa[k] / 2;
+        ss_append(g, u);
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fb[k] / 2;
+        ss_append(g, u);
+#endif
+#endif
+        tf += dt;
+    }
+}
+
+// Rotating wave of a coarse step, :823-829.
+static __device__ __forceinline__ double ss_rotating_wave(double tc, double dt_out, const double* sweep,
+                                                          const ss_ctx* ctx)
+{
+    double rot = 0.0;
+#if SS_ROT
+    double f[SS_FIELD_LEN];
+    ss_call_field(tc + dt_out / 2, sweep, f, ctx);
+    rot = f[2];
+#endif
+#if SS_DIM == 2
+    rot /= 2;
+#endif
+    return rot;
+}
+
+// Ordered combine of the G lanes of a group: rank r (a multiple of 2s) takes
+// the product of the later ranks from lane r + s, u_r <- u_{r+s} u_r.
+static __device__ __forceinline__ void ss_group_reduce(ss_cd u[SS_D][SS_D], int rank, int lanes)
+{
+#pragma unroll 1
+    for (int s = 1; s < lanes; s <<= 1) {
+        ss_cd p[SS_D][SS_D];
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) {
+                p[r][c].x = __shfl_down(u[r][c].x, s, 64);
+                p[r][c].y = __shfl_down(u[r][c].y, s, 64);
+            }
+        if ((rank & (2 * s - 1)) == 0 && rank + s < lanes) ss_left_multiply<SS_D>(p, u);
+    }
+}
+
+// Leave the rotating frame and store U and time, :838-852, :818.
+static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot, double dt_out, double tc,
+                                                long long row, double* time_out, double* u_out)
+{
+#if SS_ROT
+    double s, c;
+    sincos(rot * dt_out, &s, &c);
+    const ss_cd w = ss_c(c, s);
+    const ss_cd wc = ss_conj(w);
+#pragma unroll
+    for (int k = 0; k < SS_D; ++k) {
+        u[0][k] = ss_mul(u[0][k], wc);
+        u[SS_D - 1][k] = ss_mul(u[SS_D - 1][k], w);
+    }
+#endif
+    double2* dst = reinterpret_cast<double2*>(u_out) + row * (SS_D * SS_D);
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) dst[r * SS_D + c] = make_double2(u[r][c].x, u[r][c].y);
+    time_out[row] = tc;
+}
+
+// One simulation: get_time_evolution, spinsim/__init__.py:854-896.
+extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te_args a)
+{
+    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long long wave = gtid >> 6;
+    const int lanes = a.lanes_per_step;
+    const int groups = 64 / lanes;
+    const int gi = lane / lanes;
+    const int rank = lane - gi * lanes;
+    const long long row = wave * groups + gi;
+    const bool active = gi < groups && row < a.n_coarse;
+    const int base = a.n_fine / lanes;
+    const int rem = a.n_fine - base * lanes;
+    const int j0 = rank * base + (rank < rem ? rank : rem);
+    const int count = active ? base + (rank < rem ? 1 : 0) : 0;
+
+    ss_ctx ctx;
+    ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
+    ctx.sim = 0;
+    const double tc = a.t0 + a.dt_out * (double)row;
+    const double rot = active ? ss_rotating_wave(tc, a.dt_out, a.sweep, &ctx) : 0.0;
+
+    ss_cd u[SS_D][SS_D];
+    ss_integrate(u, j0, count, tc, rot, a.dt, a.off_a, a.off_b, a.sweep, &ctx);
+    ss_group_reduce(u, rank, lanes);
+    if (active && rank == 0) ss_store(u, rot, a.dt_out, tc, row, a.time_out, a.u_out);
+}
+
+// Many simulations in one persistent launch.  Each wave repeatedly takes
+// `items_per_grab` groups' worth of coarse rows from a global counter (one
+// atomic per wave), so simulations with 50x different lengths finish
+// together; every wave leaves once the counter passes the total.
+extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel(ss_batch_args a)
+{
+    const int lane = threadIdx.x & 63;
+    const int lanes = a.lanes_per_step;
+    const int groups = 64 / lanes;
+    const int gi = lane / lanes;
+    const int rank = lane - gi * lanes;
+    const long long grab = (long long)groups * a.items_per_grab;
+#pragma unroll 1
+    for (;;) {
+        unsigned long long start = 0;
+        if (lane == 0) start = atomicAdd(a.counter, (unsigned long long)grab);
+        start = __shfl(start, 0, 64);
+        if ((long long)start >= a.total) break;
+#pragma unroll 1
+        for (int it = 0; it < a.items_per_grab; ++it) {
+            const long long row = (long long)start + (long long)it * groups + gi;
+            const bool active = gi < groups && row < a.total;
+            // simulation owning `row`: largest s with offsets[s] <= row
+            long long lo = 0, hi = a.n_sims;
+            if (active) {
+#pragma unroll 1
+                while (hi - lo > 1) {
+                    const long long mid = (lo + hi) >> 1;
+                    if (a.offsets[mid] <= row) lo = mid; else hi = mid;
+                }
+            }
+            const int sim = (int)lo;
+            const long long local = active ? row - a.offsets[sim] : 0;
+            const int n_fine = active ? a.n_fine[sim] : 0;
+            const double dt = active ? a.dt[sim] : 0.0;
+            const double dt_out = active ? a.dt_out[sim] : 0.0;
+            const double t0 = active ? a.t0[sim] : 0.0;
+            const double* sweep = a.sweep + (long long)sim * a.sweep_stride;
+            const int base = n_fine / lanes;
+            const int rem = n_fine - base * lanes;
+            const int j0 = rank * base + (rank < rem ? rank : rem);
+            const int count = active ? base + (rank < rem ? 1 : 0) : 0;
+            ss_ctx ctx;
+            ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
+            ctx.sim = sim;
+            const double tc = t0 + dt_out * (double)local;
+            const double rot = active ? ss_rotating_wave(tc, dt_out, sweep, &ctx) : 0.0;
+            const double half = 0.5 * dt;
+            ss_cd u[SS_D][SS_D];
+            ss_integrate(u, j0, count, tc, rot, dt, half * a.off_a_frac, half * a.off_b_frac, sweep, &ctx);
+            ss_group_reduce(u, rank, lanes);
+            if (active && rank == 0) ss_store(u, rot, dt_out, tc, row, a.time_out, a.u_out);
+        }
+    }
+}

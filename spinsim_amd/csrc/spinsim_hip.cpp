@@ -1,0 +1,716 @@
+// libspinsim_hip.so: the C ABI declared in include/spinsim_hip.h.
+//
+// The time-evolution kernels depend on the user's field callback, so they
+// are compiled per callback with hipRTC for gfx950 (the kernel text is
+// embedded in this library).  The state scan and Bloch-vector kernels do not,
+// and are compiled ahead of time into this library.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "spinsim_hip.h"
+#include "device/ss_args.h"
+#include "device/ss_complex.h"
+#include "ss_embedded.h"
+
+namespace dev {  // the sampler as field callbacks see it (same layout as ::ss_table)
+#include "device/ss_prelude.h"
+}  // generated: ss_src_args, ss_src_complex, ss_src_prelude, ss_src_integrator
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const char* fmt, ...)
+{
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return 1;
+}
+
+#define SS_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+constexpr int kAbiVersion = 1;
+static_assert(sizeof(::ss_table) == 40, "ss_table layout");
+constexpr int kScanChunk = 8;   // matrices per lane in the state scan
+constexpr int kScanBlock = 256;
+
+uint64_t fnv1a(const char* s, uint64_t h)
+{
+    for (; *s; ++s) {
+        h ^= (unsigned char)*s;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+}  // namespace
+
+struct ss_module {
+    hipModule_t module = nullptr;
+    hipFunction_t single = nullptr;
+    hipFunction_t batch = nullptr;
+    int dim = 2;
+    int method = 0;
+    int expo = 0;
+    int rotating = 1;
+    int device = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Ahead-of-time kernels: state prefix product and Bloch vectors
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int D>
+__device__ __forceinline__ void load_mat(const double* u, long long k, ss_cd m[D][D])
+{
+    const double2* p = reinterpret_cast<const double2*>(u) + k * (D * D);
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const double2 v = p[r * D + c];
+            m[r][c] = ss_c(v.x, v.y);
+        }
+}
+
+template <int D>
+__device__ __forceinline__ void store_mat(double* u, long long k, const ss_cd m[D][D])
+{
+    double2* p = reinterpret_cast<double2*>(u) + k * (D * D);
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) p[r * D + c] = make_double2(m[r][c].x, m[r][c].y);
+}
+
+template <int D>
+__device__ __forceinline__ void load_vec(const double* s, long long k, ss_cd v[D])
+{
+    const double2* p = reinterpret_cast<const double2*>(s) + k * D;
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        const double2 t = p[r];
+        v[r] = ss_c(t.x, t.y);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void store_vec(double* s, long long k, const ss_cd v[D])
+{
+    double2* p = reinterpret_cast<double2*>(s) + k * D;
+#pragma unroll
+    for (int r = 0; r < D; ++r) p[r] = make_double2(v[r].x, v[r].y);
+}
+
+template <int D>
+__device__ __forceinline__ void shfl_up_mat(const ss_cd m[D][D], ss_cd out[D][D], int s)
+{
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            out[r][c].x = __shfl_up(m[r][c].x, s, 64);
+            out[r][c].y = __shfl_up(m[r][c].y, s, 64);
+        }
+}
+
+// Inclusive scan over the 64 lanes of a wave of x_l (later lanes on the
+// left): q_l = x_l x_{l-1} ... x_0.  Then ex = q_{l-1} (identity on lane 0).
+template <int D>
+__device__ __forceinline__ void wave_scan(ss_cd q[D][D], ss_cd ex[D][D], int lane)
+{
+#pragma unroll 1
+    for (int s = 1; s < 64; s <<= 1) {
+        ss_cd p[D][D];
+        shfl_up_mat<D>(q, p, s);
+        if (lane >= s) ss_right_multiply<D>(q, p);
+    }
+    shfl_up_mat<D>(q, ex, 1);
+    if (lane == 0) ss_set_identity<D>(ex);
+}
+
+// Workgroup-wide exclusive scan (256 lanes = 4 waves).  On return `ex` is
+// the product of the x of all earlier lanes of the workgroup and `total`
+// (valid on every lane) the product over the whole workgroup.
+template <int D>
+__device__ __forceinline__ void block_scan(const ss_cd x[D][D], ss_cd ex[D][D], ss_cd total[D][D])
+{
+    __shared__ ss_cd wave_tot[kScanBlock / 64][D][D];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    ss_cd q[D][D];
+    ss_copy<D>(x, q);
+    ss_cd wex[D][D];
+    wave_scan<D>(q, wex, lane);
+    if (lane == 63) ss_copy<D>(q, wave_tot[wave]);
+    __syncthreads();
+    ss_cd pre[D][D];
+    ss_set_identity<D>(pre);
+    for (int w = 0; w < wave; ++w) ss_left_multiply<D>(wave_tot[w], pre);
+    ss_matmul<D>(wex, pre, ex);
+    ss_set_identity<D>(total);
+    for (int w = 0; w < kScanBlock / 64; ++w) ss_left_multiply<D>(wave_tot[w], total);
+}
+
+// Pass 1: lane L covers states [L*C, L*C + C) and the matrices it applies,
+// [L*C, min(L*C + C, n - 1)).  Store each lane's exclusive in-block prefix and
+// each block's total.
+template <int D>
+__global__ void __launch_bounds__(kScanBlock) scan_lanes(const double* u, long long n, double* lane_prefix,
+                                                           double* block_total)
+{
+    const long long lane_id = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+    const long long k0 = lane_id * kScanChunk;
+    const long long kend = k0 + kScanChunk < n - 1 ? k0 + kScanChunk : n - 1;
+    ss_cd x[D][D];
+    ss_set_identity<D>(x);
+    for (long long k = k0; k < kend; ++k) {
+        ss_cd m[D][D];
+        load_mat<D>(u, k, m);
+        ss_left_multiply<D>(m, x);
+    }
+    ss_cd ex[D][D], total[D][D];
+    block_scan<D>(x, ex, total);
+    store_mat<D>(lane_prefix, lane_id, ex);
+    if (threadIdx.x == 0) store_mat<D>(block_total, blockIdx.x, total);
+}
+
+// Pass 2 (one workgroup): carry-in state of every block,
+// v_b = (B_{b-1} ... B_0) psi0.
+template <int D>
+__global__ void __launch_bounds__(kScanBlock) scan_blocks(const double* block_total, long long nb,
+                                                            ss_cd p0, ss_cd p1, ss_cd p2, double* carry)
+{
+    const long long per = (nb + kScanBlock - 1) / kScanBlock;
+    const long long b0 = per * threadIdx.x;
+    const long long b1 = b0 + per < nb ? b0 + per : nb;
+    ss_cd x[D][D];
+    ss_set_identity<D>(x);
+    for (long long b = b0; b < b1; ++b) {
+        ss_cd m[D][D];
+        load_mat<D>(block_total, b, m);
+        ss_left_multiply<D>(m, x);
+    }
+    ss_cd ex[D][D], total[D][D];
+    block_scan<D>(x, ex, total);
+    ss_cd psi[3] = {p0, p1, p2};
+    ss_cd v[D];
+    ss_matvec<D>(ex, psi, v);
+    for (long long b = b0; b < b1; ++b) {
+        store_vec<D>(carry, b, v);
+        ss_cd m[D][D], t[D];
+        load_mat<D>(block_total, b, m);
+        ss_matvec<D>(m, v, t);
+#pragma unroll
+        for (int r = 0; r < D; ++r) v[r] = t[r];
+    }
+}
+
+// Pass 3: each lane starts from (its in-block prefix) * (block carry) and
+// walks its chunk, writing the states.
+template <int D>
+__global__ void __launch_bounds__(kScanBlock) scan_apply(const double* u, long long n, const double* lane_prefix,
+                                                           const double* carry, double* state)
+{
+    const long long lane_id = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+    const long long k0 = lane_id * kScanChunk;
+    if (k0 >= n) return;
+    const long long kend = k0 + kScanChunk < n ? k0 + kScanChunk : n;
+    ss_cd pre[D][D], c[D], v[D];
+    load_mat<D>(lane_prefix, lane_id, pre);
+    load_vec<D>(carry, blockIdx.x, c);
+    ss_matvec<D>(pre, c, v);
+    for (long long k = k0; k < kend; ++k) {
+        store_vec<D>(state, k, v);
+        if (k + 1 < kend) {
+            ss_cd m[D][D], t[D];
+            load_mat<D>(u, k, m);
+            ss_matvec<D>(m, v, t);
+#pragma unroll
+            for (int r = 0; r < D; ++r) v[r] = t[r];
+        }
+    }
+}
+
+// One wave per simulation of a batch: the same three phases within a wave.
+template <int D>
+__global__ void __launch_bounds__(64) batch_scan(const double* psi0, const double* u, double* state,
+                                                   const long long* offsets)
+{
+    const long long sim = blockIdx.x;
+    const int lane = threadIdx.x;
+    const long long o0 = offsets[sim];
+    const long long n = offsets[sim + 1] - o0;
+    if (n <= 0) return;
+    const long long chunk = (n + 63) / 64;
+    const long long k0 = lane * chunk;
+    const long long kend_m = k0 + chunk < n - 1 ? k0 + chunk : n - 1;
+    const long long kend_s = k0 + chunk < n ? k0 + chunk : n;
+    ss_cd x[D][D];
+    ss_set_identity<D>(x);
+    for (long long k = k0; k < kend_m; ++k) {
+        ss_cd m[D][D];
+        load_mat<D>(u, o0 + k, m);
+        ss_left_multiply<D>(m, x);
+    }
+    ss_cd ex[D][D];
+    wave_scan<D>(x, ex, lane);
+    ss_cd p[D], v[D];
+    load_vec<D>(psi0, sim, p);
+    ss_matvec<D>(ex, p, v);
+    for (long long k = k0; k < kend_s; ++k) {
+        store_vec<D>(state, o0 + k, v);
+        if (k + 1 < kend_s) {
+            ss_cd m[D][D], t[D];
+            load_mat<D>(u, o0 + k, m);
+            ss_matvec<D>(m, v, t);
+#pragma unroll
+            for (int r = 0; r < D; ++r) v[r] = t[r];
+        }
+    }
+}
+
+// get_spin, spinsim/__init__.py:936-944.
+template <int D>
+__global__ void __launch_bounds__(256) spin_kernel(const double* state, double* spin, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ss_cd s[D];
+    load_vec<D>(state, i, s);
+    double sx, sy, sz;
+    if (D == 2) {
+        const ss_cd p = ss_mul(s[0], ss_conj(s[1]));
+        sx = p.x;
+        sy = -p.y;  // Re(i psi0 psi1*)
+        sz = 0.5 * (s[0].x * s[0].x + s[0].y * s[0].y - s[1].x * s[1].x - s[1].y * s[1].y);
+    } else {
+        const double r2 = 1.4142135623730951;
+        const ss_cd c1 = ss_conj(s[1]);
+        const ss_cd a = ss_mul(c1, ss_add(s[0], s[D - 1]));
+        const ss_cd b = ss_mul(c1, ss_c(s[0].x - s[D - 1].x, s[0].y - s[D - 1].y));
+        sx = 2.0 * a.x / r2;
+        sy = -2.0 * b.y / r2;  // Re(2i b)/sqrt2
+        sz = s[0].x * s[0].x + s[0].y * s[0].y - s[D - 1].x * s[D - 1].x - s[D - 1].y * s[D - 1].y;
+    }
+    spin[3 * i + 0] = sx;
+    spin[3 * i + 1] = sy;
+    spin[3 * i + 2] = sz;
+}
+
+// Evaluate sampler k of `tables` at n times (row `sim` of a batched table).
+__global__ void __launch_bounds__(256) interp_kernel(const dev::ss_table* tables, int k, int sim, const double* t,
+                                                     double* out, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dev::ss_ctx ctx;
+    ctx.tables = tables;
+    ctx.sim = sim;
+    out[i] = dev::ss_interp_sample(&ctx, k, t[i]);
+}
+
+__global__ void zero_counter(unsigned long long* p)
+{
+    if (threadIdx.x == 0) p[0] = 0ull;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct ScanLayout {
+    long long lanes, blocks;
+    size_t prefix_off, total_off, carry_off, bytes;
+};
+
+ScanLayout scan_layout(int64_t n, int dim)
+{
+    ScanLayout l{};
+    l.lanes = (n + kScanChunk - 1) / kScanChunk;
+    l.blocks = (l.lanes + kScanBlock - 1) / kScanBlock;
+    const size_t mat = size_t(dim) * dim * 16;
+    l.prefix_off = 0;
+    l.total_off = align256(size_t(l.blocks) * kScanBlock * mat);
+    l.carry_off = l.total_off + align256(size_t(l.blocks) * mat);
+    l.bytes = l.carry_off + align256(size_t(l.blocks) * dim * 16);
+    return l;
+}
+
+int default_lanes(int64_t n_coarse, int n_fine)
+{
+    // Aim for about eight waves per SIMD of an MI355X (256 CUs x 4 SIMDs).
+    const long long target = 256LL * 4 * 8 * 64;
+    int g = 1;
+    while (g < 64 && 2 * g <= n_fine && n_coarse * g < target) g *= 2;
+    return g;
+}
+
+std::string build_source(const char* field_src, int dim, int method, int expo, int rotating,
+                         int number_of_squares, int field_has_context)
+{
+    int hyper = (number_of_squares + 1) / 2;  // ceil(squares / 2), :1547
+    if (number_of_squares <= 0) hyper = 0;
+    std::string s;
+    char head[512];
+    snprintf(head, sizeof(head),
+             "#define SS_DIM %d\n#define SS_LIE %d\n#define SS_METHOD %d\n#define SS_EXPO %d\n"
+             "#define SS_ROT %d\n#define SS_HYPER %d\n#define SS_CTX %d\n",
+             dim, dim == 2 ? 3 : 4, method, expo, rotating ? 1 : 0, hyper, field_has_context ? 1 : 0);
+    s += head;
+    s += ss_src_args;
+    s += ss_src_complex;
+    s += ss_src_prelude;
+    s += "\n#line 1 \"get_field\"\n";
+    s += field_src;
+    s += "\n#line 1 \"ss_integrator.h\"\n";
+    s += ss_src_integrator;
+    return s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* ss_last_error(void) { return g_error.c_str(); }
+
+int ss_abi_version(void) { return kAbiVersion; }
+
+uint64_t ss_source_hash(void)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = fnv1a(ss_src_args, h);
+    h = fnv1a(ss_src_complex, h);
+    h = fnv1a(ss_src_prelude, h);
+    h = fnv1a(ss_src_integrator, h);
+    return h;
+}
+
+void ss_free(void* p) { std::free(p); }
+
+int ss_compile_code(const char* field_src, int dim, int method, int expo, int rotating,
+                    int number_of_squares, int field_has_context, void** code, size_t* code_size,
+                    char** log)
+{
+    if (code) *code = nullptr;
+    if (code_size) *code_size = 0;
+    if (log) *log = nullptr;
+    if (!field_src || !code || !code_size) return fail("ss_compile_code: null argument");
+    if (dim != 2 && dim != 3) return fail("ss_compile_code: dim must be 2 or 3, got %d", dim);
+    if (method < 0 || method > 2) return fail("ss_compile_code: bad method %d", method);
+    if (expo != 0 && expo != 1) return fail("ss_compile_code: bad exponentiation method %d", expo);
+    if (expo == 0 && dim != 2) return fail("ss_compile_code: analytic exponentiation is spin-half only");
+    if (number_of_squares > 60) return fail("ss_compile_code: number_of_squares %d too large", number_of_squares);
+    const std::string src = build_source(field_src, dim, method, expo, rotating, number_of_squares,
+                                         field_has_context);
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "spinsim_field.hip", 0, nullptr, nullptr) !=
+        HIPRTC_SUCCESS)
+        return fail("hiprtcCreateProgram failed");
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops", "-Wno-pragma-once-outside-header"};
+    hiprtcResult rc = hiprtcCompileProgram(prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    std::string text(log_size, '\0');
+    if (log_size) hiprtcGetProgramLog(prog, &text[0]);
+    if (log) {
+        *log = static_cast<char*>(std::malloc(text.size() + 1));
+        if (*log) std::memcpy(*log, text.c_str(), text.size() + 1);
+    }
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail("hipRTC could not compile get_field with the integrator: %s\n%s",
+                    hiprtcGetErrorString(rc), text.c_str());
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    void* buf = std::malloc(n);
+    if (!buf) {
+        hiprtcDestroyProgram(&prog);
+        return fail("out of host memory");
+    }
+    hiprtcGetCode(prog, static_cast<char*>(buf));
+    hiprtcDestroyProgram(&prog);
+    *code = buf;
+    *code_size = n;
+    return 0;
+}
+
+int ss_module_load(const void* code, size_t code_size, int dim, int method, int expo, int rotating,
+                   ss_module** out)
+{
+    if (!code || !code_size || !out) return fail("ss_module_load: null argument");
+    *out = nullptr;
+    ss_module* m = new ss_module();
+    m->dim = dim;
+    m->method = method;
+    m->expo = expo;
+    m->rotating = rotating;
+    hipError_t e = hipGetDevice(&m->device);
+    if (e == hipSuccess) e = hipModuleLoadData(&m->module, code);
+    if (e == hipSuccess) e = hipModuleGetFunction(&m->single, m->module, "ss_time_evolution_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&m->batch, m->module, "ss_batch_time_evolution_kernel");
+    if (e != hipSuccess) {
+        if (m->module) (void)hipModuleUnload(m->module);
+        delete m;
+        return fail("ss_module_load: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return 0;
+}
+
+int ss_compile(const char* field_src, int dim, int method, int expo, int rotating, int number_of_squares,
+               int field_has_context, ss_module** out)
+{
+    void* code = nullptr;
+    size_t size = 0;
+    char* log = nullptr;
+    int rc = ss_compile_code(field_src, dim, method, expo, rotating, number_of_squares, field_has_context,
+                             &code, &size, &log);
+    std::free(log);
+    if (rc) return rc;
+    rc = ss_module_load(code, size, dim, method, expo, rotating, out);
+    std::free(code);
+    return rc;
+}
+
+int ss_module_free(ss_module* m)
+{
+    if (!m) return 0;
+    hipError_t e = hipSuccess;
+    if (m->module) e = hipModuleUnload(m->module);
+    delete m;
+    if (e != hipSuccess) return fail("hipModuleUnload: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int ss_module_info(const ss_module* m, int* vgprs, int* sgprs, int* scratch_bytes, int* max_threads)
+{
+    if (!m) return fail("ss_module_info: null module");
+    int v = -1, l = -1, t = -1;
+    SS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, m->single));
+    SS_HIP(hipFuncGetAttribute(&l, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, m->single));
+    SS_HIP(hipFuncGetAttribute(&t, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, m->single));
+    if (vgprs) *vgprs = v;
+    if (sgprs) *sgprs = -1;
+    if (scratch_bytes) *scratch_bytes = l;
+    if (max_threads) *max_threads = t;
+    return 0;
+}
+
+static void sample_offsets(int method, double dt, double* off_a, double* off_b)
+{
+    // Sample points within a fine step, in the reference's operation order:
+    // CF4 t + 0.5*dt*(1 -+ 1/sqrt3) (:644, :649), midpoint t + 0.5*dt (:771),
+    // half step t and t + dt (:713, :718).
+    const double sqrt3 = 1.7320508075688772;
+    if (method == SS_METHOD_MAGNUS_CF4) {
+        *off_a = 0.5 * dt * (1 - 1 / sqrt3);
+        *off_b = 0.5 * dt * (1 + 1 / sqrt3);
+    } else if (method == SS_METHOD_MIDPOINT_SAMPLE) {
+        *off_a = 0.5 * dt;
+        *off_b = 0.0;
+    } else {
+        *off_a = 0.0;
+        *off_b = dt;
+    }
+}
+
+int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double dt, double dt_out,
+                      int64_t n_coarse, double* time_out, double* u_out, const ss_table* tables,
+                      int lanes_per_step, ss_stream stream)
+{
+    if (!m) return fail("ss_time_evolution: null module");
+    if (n_coarse < 0) return fail("ss_time_evolution: negative n_coarse");
+    if (n_coarse == 0) return 0;
+    if (!sweep || !time_out || !u_out) return fail("ss_time_evolution: null buffer");
+    if (!(dt > 0) || !(dt_out > 0)) return fail("ss_time_evolution: steps must be positive");
+    const double nf = std::floor(dt_out / dt + 0.5);  // :832
+    if (!(nf >= 0) || nf > 2147483647.0) return fail("ss_time_evolution: bad fine-step count");
+    const int n_fine = int(nf);
+    if (lanes_per_step <= 0) lanes_per_step = default_lanes(n_coarse, n_fine);
+    if (lanes_per_step > 64) return fail("ss_time_evolution: lanes_per_step must be <= 64");
+    ss_te_args a;
+    a.sweep = sweep;
+    a.tables = tables;
+    a.time_out = time_out;
+    a.u_out = u_out;
+    a.n_coarse = n_coarse;
+    a.t0 = t0;
+    a.dt = dt;
+    a.dt_out = dt_out;
+    sample_offsets(m->method, dt, &a.off_a, &a.off_b);
+    a.n_fine = n_fine;
+    a.lanes_per_step = lanes_per_step;
+    const long long groups = 64 / lanes_per_step;
+    const long long waves = (n_coarse + groups - 1) / groups;
+    const long long blocks = (waves + 3) / 4;
+    if (blocks > 2147483647LL) return fail("ss_time_evolution: grid too large");
+    void* params[] = {&a};
+    SS_HIP(hipModuleLaunchKernel(m->single, unsigned(blocks), 1, 1, 256, 1, 1, 0,
+                                 static_cast<hipStream_t>(stream), params, nullptr));
+    return 0;
+}
+
+int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t sweep_stride, int64_t n_sims,
+                            const int64_t* offsets, const double* t0, const double* dt, const double* dt_out,
+                            const int32_t* n_fine, int64_t total, double* time_out, double* u_out,
+                            const ss_table* tables, uint64_t* counter, int lanes_per_step, int items_per_grab,
+                            int grid_blocks, ss_stream stream)
+{
+    if (!m) return fail("ss_batch_time_evolution: null module");
+    if (n_sims < 0 || total < 0) return fail("ss_batch_time_evolution: negative size");
+    if (n_sims == 0 || total == 0) return 0;
+    if (!sweep || !offsets || !t0 || !dt || !dt_out || !n_fine || !time_out || !u_out || !counter)
+        return fail("ss_batch_time_evolution: null buffer");
+    if (lanes_per_step <= 0) lanes_per_step = 1;
+    if (lanes_per_step > 64) return fail("ss_batch_time_evolution: lanes_per_step must be <= 64");
+    if (items_per_grab <= 0) items_per_grab = 4;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (grid_blocks <= 0) {
+        int cus = 0, per_cu = 0;
+        SS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device));
+        SS_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->batch, 256, 0));
+        grid_blocks = std::max(1, cus * std::max(per_cu, 1));
+    }
+    const long long groups = 64 / lanes_per_step;
+    const long long per_block = 4LL * groups * items_per_grab;
+    const long long needed = (total + per_block - 1) / per_block;
+    if (grid_blocks > needed) grid_blocks = int(needed);
+    ss_batch_args a;
+    a.sweep = sweep;
+    a.tables = tables;
+    a.offsets = reinterpret_cast<const long long*>(offsets);
+    a.t0 = t0;
+    a.dt = dt;
+    a.dt_out = dt_out;
+    a.n_fine = n_fine;
+    a.time_out = time_out;
+    a.u_out = u_out;
+    a.counter = reinterpret_cast<unsigned long long*>(counter);
+    a.total = total;
+    a.n_sims = n_sims;
+    a.sweep_stride = sweep_stride;
+    double oa = 0, ob = 0;
+    sample_offsets(m->method, 2.0, &oa, &ob);  // offsets in units of 0.5*dt
+    a.off_a_frac = oa;
+    a.off_b_frac = ob;
+    a.lanes_per_step = lanes_per_step;
+    a.items_per_grab = items_per_grab;
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, s, a.counter);
+    SS_HIP(hipGetLastError());
+    void* params[] = {&a};
+    SS_HIP(hipModuleLaunchKernel(m->batch, unsigned(grid_blocks), 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    return 0;
+}
+
+size_t ss_state_scan_workspace(int64_t n, int dim)
+{
+    if (n <= 0 || (dim != 2 && dim != 3)) return 0;
+    return scan_layout(n, dim).bytes;
+}
+
+int ss_state_scan(const double* psi0, const double* u, double* state, int64_t n, int dim, void* workspace,
+                  size_t workspace_bytes, ss_stream stream)
+{
+    if (dim != 2 && dim != 3) return fail("ss_state_scan: dim must be 2 or 3");
+    if (n < 0) return fail("ss_state_scan: negative n");
+    if (n == 0) return 0;
+    if (!psi0 || !u || !state || !workspace) return fail("ss_state_scan: null buffer");
+    const ScanLayout l = scan_layout(n, dim);
+    if (workspace_bytes < l.bytes)
+        return fail("ss_state_scan: workspace of %zu bytes, need %zu", workspace_bytes, l.bytes);
+    if (l.blocks > 2147483647LL) return fail("ss_state_scan: too many rows");
+    char* ws = static_cast<char*>(workspace);
+    double* prefix = reinterpret_cast<double*>(ws + l.prefix_off);
+    double* total = reinterpret_cast<double*>(ws + l.total_off);
+    double* carry = reinterpret_cast<double*>(ws + l.carry_off);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const ss_cd p0 = ss_c(psi0[0], psi0[1]), p1 = ss_c(psi0[2], psi0[3]);
+    const ss_cd p2 = dim == 3 ? ss_c(psi0[4], psi0[5]) : ss_c(0, 0);
+    const dim3 grid(unsigned(l.blocks)), block(kScanBlock);
+    if (dim == 2) {
+        hipLaunchKernelGGL(scan_lanes<2>, grid, block, 0, s, u, (long long)n, prefix, total);
+        hipLaunchKernelGGL(scan_blocks<2>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
+        hipLaunchKernelGGL(scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state);
+    } else {
+        hipLaunchKernelGGL(scan_lanes<3>, grid, block, 0, s, u, (long long)n, prefix, total);
+        hipLaunchKernelGGL(scan_blocks<3>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
+        hipLaunchKernelGGL(scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state);
+    }
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_batch_state_scan(const double* psi0, const double* u, double* state, const int64_t* offsets,
+                        int64_t n_sims, int dim, ss_stream stream)
+{
+    if (dim != 2 && dim != 3) return fail("ss_batch_state_scan: dim must be 2 or 3");
+    if (n_sims < 0) return fail("ss_batch_state_scan: negative n_sims");
+    if (n_sims == 0) return 0;
+    if (!psi0 || !u || !state || !offsets) return fail("ss_batch_state_scan: null buffer");
+    if (n_sims > 2147483647LL) return fail("ss_batch_state_scan: too many simulations");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long* off = reinterpret_cast<const long long*>(offsets);
+    if (dim == 2)
+        hipLaunchKernelGGL(batch_scan<2>, dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
+    else
+        hipLaunchKernelGGL(batch_scan<3>, dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_interp_eval(const ss_table* tables, int k, int sim, const double* t, double* out, int64_t n,
+                   ss_stream stream)
+{
+    if (n < 0) return fail("ss_interp_eval: negative n");
+    if (n == 0) return 0;
+    if (!tables || !t || !out) return fail("ss_interp_eval: null buffer");
+    if (k < 0 || sim < 0) return fail("ss_interp_eval: negative table or row index");
+    const long long blocks = (n + 255) / 256;
+    if (blocks > 2147483647LL) return fail("ss_interp_eval: too many samples");
+    hipLaunchKernelGGL(interp_kernel, dim3(unsigned(blocks)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const dev::ss_table*>(tables), k, sim, t, out, (long long)n);
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_spin(const double* state, double* spin, int64_t n, int dim, ss_stream stream)
+{
+    if (dim != 2 && dim != 3) return fail("ss_spin: dim must be 2 or 3");
+    if (n < 0) return fail("ss_spin: negative n");
+    if (n == 0) return 0;
+    if (!state || !spin) return fail("ss_spin: null buffer");
+    const long long blocks = (n + 255) / 256;
+    if (blocks > 2147483647LL) return fail("ss_spin: too many rows");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dim == 2)
+        hipLaunchKernelGGL(spin_kernel<2>, dim3(unsigned(blocks)), dim3(256), 0, s, state, spin, (long long)n);
+    else
+        hipLaunchKernelGGL(spin_kernel<3>, dim3(unsigned(blocks)), dim3(256), 0, s, state, spin, (long long)n);
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

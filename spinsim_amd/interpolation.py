@@ -1,0 +1,126 @@
+"""Interpolation samplers: generate_interpolation_sampler (spinsim/__init__.py:1941-2147).
+
+The reference freezes (time, amplitude) into the numba-compiled sampler as
+constants, so new samples mean a recompilation.  Here a sampler is a pair of
+device arrays described by an ``ss_table`` record; the integrator reads them
+at run time (``SS_SAMPLE(k, t)`` in the generated field code), and
+``update()`` swaps in new amplitudes without recompiling.  Evaluation
+semantics (uniform-grid index guess, bisection, zero-order hold / linear /
+cubic Hermite with averaged finite-difference tangents, zero or clamped
+boundaries) are those of :2026-2142.
+
+A 2-D amplitude array (B, M) -- which the reference rejects (:2062-2063) --
+holds one row per simulation of a batch; a single simulation reads row 0.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+
+class SsTable(ctypes.Structure):
+    """Host image of ``ss_table`` (include/spinsim_hip.h)."""
+
+    _fields_ = [("time", ctypes.c_uint64), ("amplitude", ctypes.c_uint64),
+                ("size", ctypes.c_int64), ("amplitude_stride", ctypes.c_int64),
+                ("order", ctypes.c_int32), ("zero_boundary", ctypes.c_int32)]
+
+
+class InterpolationSampler:
+    """Field source interpolated from (time, amplitude) samples."""
+
+    def __init__(self, time, amplitude, order: int = 3, zero_boundary: bool = True):
+        time = np.array(time, dtype=np.float64, copy=True)
+        amplitude = np.array(amplitude, dtype=np.float64, copy=True)
+        if time.ndim != 1 or time.size < 2:
+            raise ValueError("time must be a 1-D array of at least two samples")
+        if amplitude.ndim not in (1, 2) or amplitude.shape[-1] != time.size:
+            raise ValueError("amplitude must have shape (M,) or (B, M) with M = time.size")
+        if order not in (0, 1, 3):
+            raise NotImplementedError(
+                f"Interpolation order {order} is not implemented. Try order 0 (zero order hold), "
+                "1 (linear) or 3 (cubic).")
+        self.time = time
+        self.amplitude = amplitude
+        self.order = int(order)
+        self.zero_boundary = bool(zero_boundary)
+        self.version = 0
+        self._device = {}
+
+    @property
+    def rows(self) -> int:
+        return 1 if self.amplitude.ndim == 1 else self.amplitude.shape[0]
+
+    def update(self, amplitude) -> None:
+        """Replace the amplitudes (same shape); no recompilation needed."""
+        amplitude = np.array(amplitude, dtype=np.float64, copy=True)
+        if amplitude.shape != self.amplitude.shape:
+            raise ValueError(f"amplitude shape {amplitude.shape} != {self.amplitude.shape}")
+        self.amplitude = amplitude
+        self.version += 1
+        self._device.clear()
+
+    def device_arrays(self, device):
+        """(time, amplitude) as float64 tensors on ``device`` (cached)."""
+        import torch
+
+        key = (str(device), self.version)
+        if key not in self._device:
+            self._device.clear()
+            t = torch.as_tensor(self.time, device=device)
+            a = torch.as_tensor(self.amplitude, device=device).contiguous()
+            self._device[key] = (t, a)
+        return self._device[key]
+
+    def table_record(self, device) -> SsTable:
+        t, a = self.device_arrays(device)
+        rec = SsTable()
+        rec.time = t.data_ptr()
+        rec.amplitude = a.data_ptr()
+        rec.size = self.time.size
+        rec.amplitude_stride = self.time.size if self.amplitude.ndim == 2 else 0
+        rec.order = self.order
+        rec.zero_boundary = int(self.zero_boundary)
+        return rec
+
+    def __call__(self, time_sample, row: int = 0):
+        """Evaluate on the GPU at the given time(s), like the reference's host sampler."""
+        import torch
+
+        device = torch.device("cuda", torch.cuda.current_device())
+        t = torch.as_tensor(np.atleast_1d(np.asarray(time_sample, np.float64)), device=device)
+        out = torch.empty_like(t)
+        tables = build_table_array([self], device)
+        stream = torch.cuda.current_stream(device).cuda_stream
+        _native.check(_native.lib().ss_interp_eval(tables.data_ptr(), 0, int(row), t.data_ptr(),
+                                                   out.data_ptr(), t.numel(), stream),
+                      "ss_interp_eval")
+        res = out.cpu().numpy()
+        return res[0] if np.ndim(time_sample) == 0 else res.reshape(np.shape(time_sample))
+
+
+def build_table_array(samplers, device):
+    """A device byte tensor holding ss_table records for `samplers` (in slot order)."""
+    import torch
+
+    arr = (SsTable * max(len(samplers), 1))()
+    keep = []
+    for k, s in enumerate(samplers):
+        arr[k] = s.table_record(device)
+        keep.append(s.device_arrays(device))
+    host = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+    dev = host.to(device)
+    dev._ss_keep = keep  # the tables point into these tensors
+    return dev
+
+
+def generate_interpolation_sampler(time, amplitude, order=3, zero_boundary=True, device=None):
+    """Build a sampler usable inside a field function (spinsim/__init__.py:1941).
+
+    ``device`` is accepted for drop-in compatibility; samplers always live
+    on the HIP device and are callable from the host.
+    """
+    return InterpolationSampler(time, amplitude, order=order, zero_boundary=zero_boundary)

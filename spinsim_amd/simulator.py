@@ -1,0 +1,377 @@
+"""Drop-in ``Simulator`` / ``Results`` (spinsim/__init__.py:271-1111) on MI355X.
+
+``Simulator(get_field, spin_quantum_number, ...)`` compiles the field callback
+with the integrator for gfx950 (hipRTC) at construction, as the reference
+JIT-compiles with numba (:447-451, :453-991).  ``evaluate(...)`` runs three
+device steps on the current HIP stream -- the per-coarse-step time-evolution
+kernel, the parallel state prefix product and (lazily) the Bloch vectors --
+and returns numpy arrays like the reference, or device tensors on request.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import os
+import threading
+
+import numpy as np
+
+from . import _native
+from .enums import Device, ExponentiationMethod, IntegrationMethod, SpinQuantumNumber
+from .interpolation import InterpolationSampler, build_table_array
+from .transpile import transpile
+
+_RED = "\033[31m"
+_YELLOW = "\033[33m"
+_RESET = "\033[0m"
+
+_code_cache: dict = {}
+_code_lock = threading.Lock()
+
+
+def _cache_dir():
+    d = os.environ.get("SPINSIM_AMD_CACHE")
+    if d is None:
+        d = os.path.join(os.path.expanduser("~"), ".cache", "spinsim_amd")
+    return d
+
+
+def compile_field(src: str, dim: int, method: int, expo: int, rotating: bool,
+                  number_of_squares: int, has_context: bool) -> bytes:
+    """hipRTC code object for one field source and method set (memory and disk cached)."""
+    L = _native.lib()
+    key_text = "\x00".join(map(str, (src, dim, method, expo, int(rotating), number_of_squares,
+                                      int(has_context), L.ss_source_hash(), L.ss_abi_version())))
+    key = hashlib.sha256(key_text.encode()).hexdigest()[:32]
+    with _code_lock:
+        if key in _code_cache:
+            return _code_cache[key]
+    path = os.path.join(_cache_dir(), f"{key}.co")
+    code = None
+    if os.path.exists(path):
+        with open(path, "rb") as fh:
+            code = fh.read()
+    if not code:
+        code, _log = _native.compile_code(src, dim, method, expo, rotating, number_of_squares,
+                                          has_context)
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            tmp = f"{path}.tmp{os.getpid()}"
+            with open(tmp, "wb") as fh:
+                fh.write(code)
+            os.replace(tmp, path)
+        except OSError:
+            pass
+    with _code_lock:
+        _code_cache[key] = code
+    return code
+
+
+class FieldSource:
+    """A field callback as HIP-C text plus the samplers it reads.
+
+    ``get_field`` may be a Python function (translated by
+    :func:`spinsim_amd.transpile.transpile`), a string of HIP-C defining
+    ``__device__ void get_field(double t, const double* sweep, double* field)``
+    (``..., SS_CONTEXT)`` when it reads samplers), or a FieldSource.
+    ``samplers`` maps names usable in HIP-C text to samplers (or is a list,
+    read with ``SS_SAMPLE(k, t)``).
+    """
+
+    def __init__(self, get_field, samplers=None):
+        if isinstance(get_field, FieldSource):
+            self.source, self.samplers, self.has_context = (
+                get_field.source, list(get_field.samplers), get_field.has_context)
+            return
+        if isinstance(get_field, str):
+            names = []
+            if isinstance(samplers, dict):
+                names = list(samplers.keys())
+                self.samplers = list(samplers.values())
+            else:
+                self.samplers = list(samplers or [])
+            for s in self.samplers:
+                if not isinstance(s, InterpolationSampler):
+                    raise TypeError("samplers must be InterpolationSampler objects")
+            macros = "".join(f"#define {n}(t) SS_SAMPLE({k}, (t))\n" for k, n in enumerate(names))
+            self.source = macros + get_field
+            self.has_context = bool(self.samplers) or "SS_CONTEXT" in get_field
+            return
+        if callable(get_field):
+            if samplers:
+                raise TypeError("samplers= is only used with HIP-C source; Python field "
+                                "functions reference their samplers directly")
+            self.source, self.samplers, self.has_context = transpile(get_field)
+            return
+        raise TypeError("get_field must be a Python function, HIP-C source text or a FieldSource")
+
+
+class Results:
+    """The results of an evaluation (spinsim/__init__.py:271-332).
+
+    ``time`` (N,) float64, ``time_evolution`` (N, d, d) complex128,
+    ``state`` (N, d) complex128, and ``spin`` (N, 3) float64 computed on first
+    access with ``spin_calculator``.  With ``output="torch"`` these are device
+    tensors.
+    """
+
+    def __init__(self, time, time_evolution, state, spin_calculator, _device_state=None):
+        self.time = time
+        self.time_evolution = time_evolution
+        self.state = state
+        self.spin_calculator = spin_calculator
+        self._device_state = _device_state
+
+    def __getattr__(self, attr_name: str):
+        if attr_name == "spin":
+            src = self.__dict__.get("_device_state")
+            if src is not None and isinstance(self.__dict__["state"], np.ndarray):
+                spin = self.spin_calculator(src).cpu().numpy()
+            else:
+                spin = self.spin_calculator(self.__dict__["state"])
+            setattr(self, attr_name, spin)
+            return spin
+        raise AttributeError("{} has no attribute called {}.".format(self, attr_name))
+
+
+def _default_device():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return Device.ROCM
+    except ImportError:
+        pass
+    return Device.ROCM  # construction still compiles; evaluation needs a HIP device
+
+
+def resolve_exponentiation(spin_quantum_number, exponentiation_method, warn=True):
+    """Defaults and the ANALYTIC -> LIE_TROTTER fallback of :529-559."""
+    if not exponentiation_method:
+        if spin_quantum_number == SpinQuantumNumber.ONE:
+            exponentiation_method = ExponentiationMethod.LIE_TROTTER
+        else:
+            exponentiation_method = ExponentiationMethod.ANALYTIC
+    if (exponentiation_method == ExponentiationMethod.ANALYTIC
+            and spin_quantum_number != SpinQuantumNumber.HALF):
+        if warn:
+            print(f"{_RED}spinsim warning!!!\n_attempting to use an analytic exponentiation "
+                  f"method outside of spin-half. Switching to a Lie Trotter method.{_RESET}")
+        exponentiation_method = ExponentiationMethod.LIE_TROTTER
+    if exponentiation_method == ExponentiationMethod.LIE_TROTTER_8:
+        raise NotImplementedError(
+            "ExponentiationMethod.LIE_TROTTER_8 is not provided by spinsim_amd; use LIE_TROTTER")
+    return exponentiation_method
+
+
+def derive_steps(time_start, time_end, time_step_integration, time_step_output, warn=True):
+    """evaluate()'s step arithmetic, :1025-1034: (dt, N, time_end_points)."""
+    dt_old = time_step_integration
+    dt = time_step_output / round(max(time_step_output / time_step_integration, 1))
+    if math.fabs(time_step_output / dt_old - round(time_step_output / dt_old)) > 1e-6 and warn:
+        print(f"{_YELLOW}spinsim warning!!!\ntime_step_output ({time_step_output:8.4e}) not an "
+              f"integer multiple of time_step_integration ({dt_old:8.4e}). Resetting "
+              f"time_step_integration to {dt:8.4e}.{_RESET}\n")
+    ends = np.asarray([time_start, time_end], np.float64)
+    n = int((ends[1] - ends[0]) / time_step_output)
+    return float(dt), n, ends
+
+
+class Simulator:
+    """Drop-in for spinsim.Simulator (spinsim/__init__.py:334-1111).
+
+    Extra keyword arguments: ``samplers`` (for HIP-C field text) and
+    ``lanes_per_step`` (lanes sharing one coarse step; default: tuned).
+    ``threads_per_block``, ``max_registers`` and ``number_of_threads`` are
+    accepted for compatibility and do not change the gfx950 launch.
+    """
+
+    def __init__(self, get_field, spin_quantum_number: SpinQuantumNumber, device: Device = None,
+                 exponentiation_method: ExponentiationMethod = None, use_rotating_frame: bool = True,
+                 integration_method: IntegrationMethod = IntegrationMethod.MAGNUS_CF4,
+                 number_of_squares: int = 24, threads_per_block: int = 64, max_registers: int = None,
+                 number_of_threads: int = None, *, samplers=None, lanes_per_step: int = None):
+        if not device:
+            device = _default_device()
+        if device != Device.ROCM:
+            raise NotImplementedError(
+                f"{device} is a numba back end of the reference; spinsim_amd runs on ROCm "
+                "devices (Device.ROCM)")
+        self.threads_per_block = threads_per_block
+        self.spin_quantum_number = spin_quantum_number
+        self.device = device
+        self.number_of_threads = number_of_threads
+        self.use_rotating_frame = bool(use_rotating_frame)
+        self.integration_method = integration_method
+        self.number_of_squares = int(number_of_squares)
+        self.lanes_per_step = lanes_per_step
+        self.exponentiation_method = resolve_exponentiation(spin_quantum_number,
+                                                            exponentiation_method)
+        self.dimension = spin_quantum_number.dimension
+        self._modules = {}
+        self._tables = {}
+        try:
+            self.field = FieldSource(get_field, samplers)
+            self._code = compile_field(self.field.source, self.dimension,
+                                       integration_method.index, self.exponentiation_method.index,
+                                       self.use_rotating_frame, self.number_of_squares,
+                                       self.field.has_context)
+        except Exception:
+            print(f"{_RED}spinsim error!!!\nhipRTC could not compile get_field() into a device "
+                  f"function.{_RESET}\n")
+            raise
+        self.get_time_evolution = self._get_time_evolution
+        self.spin_calculator = self._spin_calculator
+
+    # ---- device plumbing --------------------------------------------------
+    @staticmethod
+    def _torch():
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("spinsim_amd: no HIP device is visible (Device.ROCM needs one)")
+        return torch
+
+    def _module(self, device):
+        key = device.index
+        if key not in self._modules:
+            torch = self._torch()
+            with torch.cuda.device(device):
+                self._modules[key] = _native.Module(
+                    self._code, self.dimension, self.integration_method.index,
+                    self.exponentiation_method.index, self.use_rotating_frame)
+        return self._modules[key]
+
+    def _table_ptr(self, device):
+        if not self.field.samplers:
+            return None
+        versions = tuple(s.version for s in self.field.samplers)
+        cached = self._tables.get(device.index)
+        if cached is None or cached[0] != versions:
+            cached = (versions, build_table_array(self.field.samplers, device))
+            self._tables[device.index] = cached
+        return cached[1].data_ptr()
+
+    def module_info(self):
+        torch = self._torch()
+        return self._module(torch.device("cuda", torch.cuda.current_device())).info()
+
+    # ---- launches on resident device buffers (no allocation, no copies) -----
+    def launch_time_evolution(self, sweep, t0, dt, dt_out, time_out, u_out, lanes_per_step=None,
+                              stream=None):
+        """Enqueue ss_time_evolution: sweep (P,) float64, time_out (N,) float64 and
+        u_out (N, d, d) complex128 are contiguous device tensors."""
+        torch = self._torch()
+        dev = u_out.device
+        n = int(time_out.shape[0])
+        lanes = lanes_per_step if lanes_per_step is not None else (self.lanes_per_step or 0)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        mod = self._module(dev)
+        _native.check(_native.lib().ss_time_evolution(
+            mod.handle, sweep.data_ptr(), float(t0), float(dt), float(dt_out), n,
+            time_out.data_ptr(), u_out.data_ptr(), self._table_ptr(dev), int(lanes), stream),
+            "ss_time_evolution")
+
+    def launch_state_scan(self, psi0, u, state, workspace, stream=None):
+        """Enqueue ss_state_scan; psi0 is a host complex128 array of d values."""
+        torch = self._torch()
+        n, d = int(u.shape[0]), self.dimension
+        psi = np.ascontiguousarray(np.asarray(psi0, np.complex128).reshape(d))
+        if stream is None:
+            stream = torch.cuda.current_stream(u.device).cuda_stream
+        _native.check(_native.lib().ss_state_scan(psi.ctypes.data, u.data_ptr(), state.data_ptr(),
+                                                  n, d, workspace.data_ptr(), workspace.numel(),
+                                                  stream), "ss_state_scan")
+
+    def launch_spin(self, state, spin, stream=None):
+        torch = self._torch()
+        n, d = int(state.shape[0]), int(state.shape[1])
+        if stream is None:
+            stream = torch.cuda.current_stream(state.device).cuda_stream
+        _native.check(_native.lib().ss_spin(state.data_ptr(), spin.data_ptr(), n, d, stream),
+                      "ss_spin")
+
+    def scan_workspace(self, n, device):
+        torch = self._torch()
+        nbytes = _native.lib().ss_state_scan_workspace(int(n), self.dimension)
+        return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+
+    # ---- reference kernels ----------------------------------------------------
+    def _get_time_evolution(self, sweep_parameters, time_coarse, time_end_points,
+                            time_step_integration, time_step_output, time_evolution_output,
+                            lanes_per_step=None):
+        """get_time_evolution(sweep, time_coarse, time_end_points, dt, dt_out, U), :854-896.
+
+        ``time_coarse`` (N,) float64 and ``time_evolution_output`` (N, d, d)
+        complex128 are device tensors filled in place (numpy arrays are copied
+        through the device).
+        """
+        torch = self._torch()
+        out_np = isinstance(time_evolution_output, np.ndarray)
+        if out_np:
+            dev = torch.device("cuda", torch.cuda.current_device())
+            t_dev = torch.empty(time_coarse.shape[0], dtype=torch.float64, device=dev)
+            u_dev = torch.empty(time_evolution_output.shape, dtype=torch.complex128, device=dev)
+        else:
+            t_dev, u_dev = time_coarse, time_evolution_output
+            dev = u_dev.device
+        n = int(t_dev.shape[0])
+        d = self.dimension
+        if tuple(u_dev.shape) != (n, d, d) or u_dev.dtype != torch.complex128 \
+                or not u_dev.is_contiguous() or t_dev.dtype != torch.float64 \
+                or not t_dev.is_contiguous() or t_dev.device != dev:
+            raise ValueError(f"time_coarse must be ({n},) float64 and time_evolution_output "
+                             f"({n}, {d}, {d}) complex128, contiguous, on one device")
+        sweep = torch.as_tensor(np.atleast_1d(np.asarray(sweep_parameters, np.float64)),
+                                device=dev)
+        t0 = float(np.asarray(time_end_points, np.float64)[0])
+        self.launch_time_evolution(sweep, t0, time_step_integration, time_step_output, t_dev,
+                                   u_dev, lanes_per_step)
+        if out_np:
+            time_coarse[...] = t_dev.cpu().numpy()
+            time_evolution_output[...] = u_dev.cpu().numpy()
+        return None
+
+    def state_scan(self, state_init, time_evolution):
+        """get_state on the device, :1087-1111: state[i] = U[i-1] state[i-1]."""
+        torch = self._torch()
+        u = time_evolution
+        n, d = int(u.shape[0]), self.dimension
+        state = torch.empty((n, d), dtype=torch.complex128, device=u.device)
+        self.launch_state_scan(state_init, u, state, self.scan_workspace(n, u.device))
+        return state
+
+    def _spin_calculator(self, state):
+        """Bloch vectors of a state time series (spin_calculator, :961-988)."""
+        torch = self._torch()
+        host = isinstance(state, np.ndarray)
+        if host:
+            st = torch.as_tensor(np.ascontiguousarray(state, np.complex128),
+                                 device=torch.device("cuda", torch.cuda.current_device()))
+        else:
+            st = state.contiguous()
+        spin = torch.empty((int(st.shape[0]), 3), dtype=torch.float64, device=st.device)
+        self.launch_spin(st, spin)
+        return spin.cpu().numpy() if host else spin
+
+    # ---- evaluate ---------------------------------------------------------
+    def evaluate(self, time_start: float, time_end: float, time_step_integration: float,
+                 time_step_output: float, state_init, sweep_parameters=[0], *,
+                 output: str = "numpy", lanes_per_step: int = None) -> Results:
+        """Integrate and return the state time series (spinsim/__init__.py:993-1085)."""
+        torch = self._torch()
+        dt, n, ends = derive_steps(time_start, time_end, time_step_integration, time_step_output)
+        if n < 0:
+            raise ValueError("time_end is before time_start")
+        d = self.dimension
+        dev = torch.device("cuda", torch.cuda.current_device())
+        time = torch.empty(n, dtype=torch.float64, device=dev)
+        u = torch.empty((n, d, d), dtype=torch.complex128, device=dev)
+        self._get_time_evolution(sweep_parameters, time, ends, dt, time_step_output, u,
+                                 lanes_per_step=lanes_per_step)
+        state = self.state_scan(state_init, u)
+        if output == "torch":
+            return Results(time, u, state, self.spin_calculator)
+        if output != "numpy":
+            raise ValueError("output must be 'numpy' or 'torch'")
+        return Results(time.cpu().numpy(), u.cpu().numpy(), state.cpu().numpy(),
+                       self.spin_calculator, _device_state=state)
