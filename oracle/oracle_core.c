@@ -32,7 +32,9 @@ static int ss_find_time_index(const ss_table *tb, double t, double *param)
     const double *time = tb->time;
     int64_t size = tb->size;
     if (time[0] <= t && t <= time[size - 1]) {
-        int64_t i = (int64_t)((t - time[0]) / (time[1] - time[0]));
+        /* guard against an undefined conversion when the guess is not finite */
+        double guess = (t - time[0]) / (time[1] - time[0]);
+        int64_t i = (guess >= 0.0 && guess < (double)size) ? (int64_t)guess : size;
         int low = 0, high = 1;
         if (i + 1 < size) {
             low = time[i] <= t;

@@ -49,7 +49,10 @@ __device__ static inline long long ss_find_time_index(const ss_table* tb, double
     const double* time = tb->time;
     const long long size = tb->size;
     if (time[0] <= t && t <= time[size - 1]) {
-        long long i = (long long)((t - time[0]) / (time[1] - time[0]));
+        // A guess outside [0, size) (a repeated first knot gives inf) goes
+        // straight to the bisection instead of an undefined conversion.
+        const double guess = (t - time[0]) / (time[1] - time[0]);
+        long long i = (guess >= 0.0 && guess < (double)size) ? (long long)guess : size;
         bool low = false, high = true;
         if (i + 1 < size) {
             low = time[i] <= t;
