@@ -292,6 +292,14 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
     }
 }
 
+// Start time of coarse step i, time_end_points[0] + time_step_output*i (:818),
+// rounded twice like the reference (no fused multiply-add).
+static __device__ __forceinline__ double ss_coarse_time(double t0, double dt_out, long long i)
+{
+#pragma clang fp contract(off)
+    return t0 + dt_out * (double)i;
+}
+
 // Rotating wave of a coarse step, :823-829.
 static __device__ __forceinline__ double ss_rotating_wave(double tc, double dt_out, const double* sweep,
                                                           const ss_ctx* ctx)
@@ -369,7 +377,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te
     ss_ctx ctx;
     ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
     ctx.sim = 0;
-    const double tc = a.t0 + a.dt_out * (double)row;
+    const double tc = ss_coarse_time(a.t0, a.dt_out, row);
     const double rot = active ? ss_rotating_wave(tc, a.dt_out, a.sweep, &ctx) : 0.0;
 
     ss_cd u[SS_D][SS_D];
@@ -423,7 +431,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
             ss_ctx ctx;
             ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
             ctx.sim = sim;
-            const double tc = t0 + dt_out * (double)local;
+            const double tc = ss_coarse_time(t0, dt_out, local);
             const double rot = active ? ss_rotating_wave(tc, dt_out, sweep, &ctx) : 0.0;
             const double half = 0.5 * dt;
             ss_cd u[SS_D][SS_D];

@@ -375,3 +375,14 @@ class Simulator:
             raise ValueError("output must be 'numpy' or 'torch'")
         return Results(time.cpu().numpy(), u.cpu().numpy(), state.cpu().numpy(),
                        self.spin_calculator, _device_state=state)
+
+    def evaluate_batch(self, time_start, time_end, time_step_integration, time_step_output,
+                       state_init, sweep_parameters, **kw):
+        """Evaluate B simulations (sweep rows) in one persistent launch.
+
+        Any of the times/steps may be per-simulation (B,) arrays and
+        ``state_init`` may be (B, d); returns :class:`spinsim_amd.batch.BatchResults`.
+        """
+        from .batch import evaluate_batch
+        return evaluate_batch(self, time_start, time_end, time_step_integration,
+                              time_step_output, state_init, sweep_parameters, **kw)
