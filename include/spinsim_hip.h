@@ -58,6 +58,12 @@ enum { /* IntegrationMethod, :68-93 */
     SS_METHOD_MIDPOINT_SAMPLE = 1, /* the reference's EULER */
     SS_METHOD_HALF_STEP = 2        /* the reference's HEUN */
 };
+enum { /* arithmetic of the time-evolution kernel (no reference counterpart) */
+    SS_PRECISION_FP64 = 0,     /* everything float64: the reference's arithmetic */
+    SS_PRECISION_FP32_BULK = 1 /* higher-order residuals of the step operators in
+                                  float32; samples, first-order terms, products
+                                  across lanes, outputs and scans float64 */
+};
 enum { /* ExponentiationMethod, :95-129 (LIE_TROTTER_8 not provided) */
     SS_EXPO_ANALYTIC = 0,
     SS_EXPO_LIE_TROTTER = 1
@@ -75,11 +81,12 @@ uint64_t ss_source_hash(void);
  *   __device__ void get_field(double t, const double *sweep, double *field)
  * or, with field_has_context = 1, the same with a trailing SS_CONTEXT
  * argument (for samplers).  dim is 2 (spin half) or 3 (spin one);
- * number_of_squares as in the reference (Lie-Trotter squarings, :1547-1550).
+ * number_of_squares as in the reference (Lie-Trotter squarings, :1547-1550);
+ * precision: SS_PRECISION_*.
  * On success *code / *code_size hold a code object and *log the compiler log
  * (both freed with ss_free); on failure *log holds the log when available. */
 int ss_compile_code(const char *field_src, int dim, int method, int expo, int rotating,
-                    int number_of_squares, int field_has_context, void **code,
+                    int number_of_squares, int field_has_context, int precision, void **code,
                     size_t *code_size, char **log);
 void ss_free(void *p);
 
@@ -88,7 +95,7 @@ int ss_module_load(const void *code, size_t code_size, int dim, int method, int 
                    int rotating, ss_module **out);
 /* ss_compile_code + ss_module_load. */
 int ss_compile(const char *field_src, int dim, int method, int expo, int rotating,
-               int number_of_squares, int field_has_context, ss_module **out);
+               int number_of_squares, int field_has_context, int precision, ss_module **out);
 int ss_module_free(ss_module *m);
 /* Resource use of the single-simulation kernel (any pointer may be NULL). */
 int ss_module_info(const ss_module *m, int *vgprs, int *sgprs, int *scratch_bytes,

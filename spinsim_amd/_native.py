@@ -25,13 +25,13 @@ SIGNATURES = {
     "ss_last_error": (ctypes.c_char_p, []),
     "ss_abi_version": (c_int, []),
     "ss_source_hash": (ctypes.c_uint64, []),
-    "ss_compile_code": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, c_int,
+    "ss_compile_code": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                 ctypes.POINTER(c_vp), ctypes.POINTER(ctypes.c_size_t),
                                 ctypes.POINTER(c_vp)]),
     "ss_free": (None, [c_vp]),
     "ss_module_load": (c_int, [c_vp, ctypes.c_size_t, c_int, c_int, c_int, c_int,
                                ctypes.POINTER(c_vp)]),
-    "ss_compile": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, c_int,
+    "ss_compile": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                            ctypes.POINTER(c_vp)]),
     "ss_module_free": (c_int, [c_vp]),
     "ss_module_info": (c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
@@ -80,14 +80,15 @@ def check(status: int, what: str) -> None:
 
 
 def compile_code(field_src: str, dim: int, method: int, expo: int, rotating: bool,
-                 number_of_squares: int, field_has_context: bool):
+                 number_of_squares: int, field_has_context: bool, precision: int = 0):
     """hipRTC-compile a field callback with the integrator; returns (code, log)."""
     L = lib()
     code = c_vp()
     size = ctypes.c_size_t()
     log = c_vp()
     rc = L.ss_compile_code(field_src.encode(), dim, method, expo, int(rotating),
-                           int(number_of_squares), int(field_has_context), ctypes.byref(code),
+                           int(number_of_squares), int(field_has_context), int(precision),
+                           ctypes.byref(code),
                            ctypes.byref(size), ctypes.byref(log))
     text = ctypes.string_at(log.value).decode(errors="replace") if log.value else ""
     if log.value:

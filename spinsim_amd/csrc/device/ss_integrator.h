@@ -7,6 +7,7 @@
 //   SS_ROT 0|1          rotating frame
 //   SS_HYPER h          Lie-Trotter hypercubes (2h residual squarings)
 //   SS_CTX 0|1          callback takes the sampler context argument
+//   SS_PREC 0|1         plain float64 / float32 bulk (see ss_integrate_mixed)
 //
 // Each coarse (output) step i integrates n_fine sub-steps into U[i]
 // (spinsim/__init__.py:796-852).  A group of G lanes shares one coarse step:
@@ -237,6 +238,224 @@ static __device__ __forceinline__ void ss_append(const double g[SS_FIELD_LEN], s
     ss_left_multiply<SS_D>(e, u);
 }
 
+#if SS_PREC == 1
+// ---- float32-bulk step operators -------------------------------------------
+// A step operator is E = exp(A), A = -i G(g).  Its first-order part A is kept
+// in float64 (as the generator vector g, summed exactly over a chunk of
+// steps); only the higher-order residual B = exp(A) - I - A and the products
+// of residuals are float32.  A chunk's product is I + A(sum g) + b with b
+// float32 of size |A|^2, so float32 rounding acts on second-order terms only.
+struct ss_cf {
+    float x, y;
+};
+static __device__ __forceinline__ ss_cf ss_f(float x, float y)
+{
+    ss_cf r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+static __device__ __forceinline__ ss_cf ss_fmac(ss_cf acc, ss_cf a, ss_cf b)
+{
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+    return acc;
+}
+// i*d*z for real d
+static __device__ __forceinline__ ss_cf ss_fimul(float d, ss_cf z) { return ss_f(-d * z.y, d * z.x); }
+
+// A = -i G(g) as float: imaginary diagonal (0, dk), upper u = A[k][k+1],
+// lower l = A[k+1][k] (both constant along the off-diagonals), A[0][2] = 0.
+struct ss_gen {
+    float d[SS_D];
+    ss_cf u, l;
+};
+static __device__ __forceinline__ ss_gen ss_generator(const double* g, double scale)
+{
+    ss_gen a;
+#if SS_DIM == 2
+    // G = g.sigma/2
+    const double hx = 0.5 * scale * g[0], hy = 0.5 * scale * g[1], hz = 0.5 * scale * g[2];
+    a.d[0] = (float)(-hz);
+    a.d[1] = (float)hz;
+    a.u = ss_f((float)(-hy), (float)(-hx));
+    a.l = ss_f((float)hy, (float)(-hx));
+#else
+    // G = gx Jx + gy Jy + gz Jz + gq Q, Q = diag(1, -2, 1)/3
+    const double rx = scale * g[0] * SS_INV_SQRT2, ry = scale * g[1] * SS_INV_SQRT2;
+    const double z = scale * g[2], q3 = scale * g[3] * (1.0 / 3.0);
+    a.d[0] = (float)(-(z + q3));
+    a.d[1] = (float)(2.0 * q3);
+    a.d[2] = (float)(z - q3);
+    a.u = ss_f((float)(-ry), (float)(-rx));
+    a.l = ss_f((float)ry, (float)(-rx));
+#endif
+    return a;
+}
+
+// out = A x
+static __device__ __forceinline__ void ss_gen_apply(const ss_gen& a, const ss_cf x[SS_D][SS_D],
+                                                    ss_cf out[SS_D][SS_D])
+{
+#pragma unroll
+    for (int c = 0; c < SS_D; ++c) {
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) {
+            ss_cf acc = ss_fimul(a.d[r], x[r][c]);
+            if (r > 0) acc = ss_fmac(acc, a.l, x[r - 1][c]);
+            if (r + 1 < SS_D) acc = ss_fmac(acc, a.u, x[r + 1][c]);
+            out[r][c] = acc;
+        }
+    }
+}
+
+// Dense out = x y (float32)
+static __device__ __forceinline__ void ss_fmatmul(const ss_cf x[SS_D][SS_D], const ss_cf y[SS_D][SS_D],
+                                                  ss_cf out[SS_D][SS_D])
+{
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) {
+            ss_cf acc = ss_f(x[r][0].x * y[0][c].x - x[r][0].y * y[0][c].y,
+                             x[r][0].x * y[0][c].y + x[r][0].y * y[0][c].x);
+#pragma unroll
+            for (int k = 1; k < SS_D; ++k) acc = ss_fmac(acc, x[r][k], y[k][c]);
+            out[r][c] = acc;
+        }
+}
+
+// Dense float copy of A.
+static __device__ __forceinline__ void ss_gen_dense(const ss_gen& a, ss_cf m[SS_D][SS_D])
+{
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c)
+            m[r][c] = (r == c) ? ss_f(0.f, a.d[r]) : (c == r + 1) ? a.u : (r == c + 1) ? a.l : ss_f(0.f, 0.f);
+}
+
+// B = exp(A) - I - A by a Taylor series (order 4 when the norm bound nu of G
+// is at most 0.015, order 6 up to 0.17: truncation below float32 resolution
+// relative to B), with scaling and squaring beyond: the residual X = A + B of
+// A/2^s is squared s times as X <- 2X + X^2, whose first-order part 2A is exact.
+static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b[SS_D][SS_D])
+{
+#if SS_DIM == 2
+    const double nu = 0.5 * (fabs(g[0]) + fabs(g[1]) + fabs(g[2]));
+#else
+    const double nu = fabs(g[0]) + fabs(g[1]) + fabs(g[2]) + fabs(g[3]);
+#endif
+    int squares = 0;
+    double scale = 1.0;
+    if (nu > 0.17) {
+        squares = (int)ceil(log2(nu / 0.17));
+        scale = ldexp(1.0, -squares);
+    }
+    const ss_gen a = ss_generator(g, scale);
+    ss_cf t[SS_D][SS_D], w[SS_D][SS_D];
+    const bool short_series = nu <= 0.015;
+    // t = c_top A + c_next I
+    const float c_top = short_series ? (1.0f / 24.0f) : (1.0f / 720.0f);
+    const float c_next = short_series ? (1.0f / 6.0f) : (1.0f / 120.0f);
+    ss_gen_dense(a, t);
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) {
+            t[r][c].x *= c_top;
+            t[r][c].y *= c_top;
+            if (r == c) t[r][c].x += c_next;
+        }
+    if (!short_series) {
+        const float cs[3] = {1.0f / 24.0f, 1.0f / 6.0f, 0.5f};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ss_gen_apply(a, t, w);
+#pragma unroll
+            for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+                for (int c = 0; c < SS_D; ++c) t[r][c] = (r == c) ? ss_f(w[r][c].x + cs[k], w[r][c].y) : w[r][c];
+        }
+    } else {
+        ss_gen_apply(a, t, w);
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) t[r][c] = (r == c) ? ss_f(w[r][c].x + 0.5f, w[r][c].y) : w[r][c];
+    }
+    ss_gen_apply(a, t, w);   // A T
+    ss_gen_apply(a, w, b);   // A A T
+#pragma unroll 1
+    for (int k = 0; k < squares; ++k) {
+        // X = A_k + B_k, A_k = 2^k A_s; new B = 2 B + X^2
+        ss_cf x[SS_D][SS_D];
+        ss_gen_dense(a, x);
+        const float f = ldexpf(1.0f, k);
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) x[r][c] = ss_f(x[r][c].x * f + b[r][c].x, x[r][c].y * f + b[r][c].y);
+        ss_fmatmul(x, x, w);
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) b[r][c] = ss_f(2.f * b[r][c].x + w[r][c].x, 2.f * b[r][c].y + w[r][c].y);
+    }
+}
+
+// P <- exp(A(g)) P for P = I + A(gsum) + b:  gsum += g,
+// b <- b + B + (A(g) + B)(A(gsum_old) + b).
+static __device__ __forceinline__ void ss_append_mixed(const double g[SS_FIELD_LEN], double gsum[SS_LIE],
+                                                       ss_cf b[SS_D][SS_D])
+{
+    ss_cf bb[SS_D][SS_D], x[SS_D][SS_D], p[SS_D][SS_D], w[SS_D][SS_D];
+    ss_expo_residual(g, bb);
+    ss_gen_dense(ss_generator(g, 1.0), x);
+    ss_gen_dense(ss_generator(gsum, 1.0), p);
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) {
+            x[r][c] = ss_f(x[r][c].x + bb[r][c].x, x[r][c].y + bb[r][c].y);
+            p[r][c] = ss_f(p[r][c].x + b[r][c].x, p[r][c].y + b[r][c].y);
+        }
+    ss_fmatmul(x, p, w);
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c)
+            b[r][c] = ss_f(b[r][c].x + bb[r][c].x + w[r][c].x, b[r][c].y + bb[r][c].y + w[r][c].y);
+#pragma unroll
+    for (int k = 0; k < SS_LIE; ++k) gsum[k] += g[k];
+}
+
+// U = I + A(gsum) + b in float64.
+static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], const ss_cf b[SS_D][SS_D],
+                                                     ss_cd u[SS_D][SS_D])
+{
+    double gg[4] = {gsum[0], gsum[1], gsum[2], SS_LIE > 3 ? gsum[SS_LIE - 1] : 0.0};
+#if SS_DIM == 2
+    const double hx = 0.5 * gg[0], hy = 0.5 * gg[1], hz = 0.5 * gg[2];
+    const double d[2] = {-hz, hz};
+    const ss_cd up = ss_c(-hy, -hx), lo = ss_c(hy, -hx);
+#else
+    const double rx = gg[0] * SS_INV_SQRT2, ry = gg[1] * SS_INV_SQRT2, z = gg[2], q3 = gg[3] * (1.0 / 3.0);
+    const double d[3] = {-(z + q3), 2.0 * q3, z - q3};
+    const ss_cd up = ss_c(-ry, -rx), lo = ss_c(ry, -rx);
+#endif
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) {
+            ss_cd a = (r == c) ? ss_c(1.0, d[r]) : (c == r + 1) ? up : (r == c + 1) ? lo : ss_c(0.0, 0.0);
+            u[r][c] = ss_c(a.x + (double)b[r][c].x, a.y + (double)b[r][c].y);
+        }
+}
+#endif
+
 // `count` fine steps starting at fine index j0 of the coarse step starting at
 // tc; u (initially identity) accumulates them.  The fine time is advanced by
 // repeated addition from tc exactly like the reference (:819, :836).
@@ -248,6 +467,19 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #pragma unroll 1
     for (int j = 0; j < j0; ++j) tf += dt;
     ss_set_identity<SS_D>(u);
+#if SS_PREC == 1
+    double gsum[SS_LIE];
+    ss_cf bres[SS_D][SS_D];
+#pragma unroll
+    for (int k = 0; k < SS_LIE; ++k) gsum[k] = 0.0;
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+        for (int c = 0; c < SS_D; ++c) bres[r][c] = ss_f(0.f, 0.f);
+#define SS_APPEND(g) ss_append_mixed(g, gsum, bres)
+#else
+#define SS_APPEND(g) ss_append(g, u)
+#endif
 #pragma unroll 1
     for (int j = 0; j < count; ++j) {
         double fa[SS_FIELD_LEN], g[SS_FIELD_LEN];
@@ -259,7 +491,7 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
         ss_frame(fa, rot, wa);
 #pragma unroll
         for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fa[k];
-        ss_append(g, u);
+        SS_APPEND(g);
 #else
         double fb[SS_FIELD_LEN];
         double tr = (tf + off_a) - tc;
@@ -274,22 +506,26 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
         // magnus_cf4, :655-686
 #pragma unroll
         for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * (SS_W0 * fa[k] + SS_W1 * fb[k]);
-        ss_append(g, u);
+        SS_APPEND(g);
 #pragma unroll
         for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * (SS_W1 * fa[k] + SS_W0 * fb[k]);
-        ss_append(g, u);
+        SS_APPEND(g);
 #else
         // heun (half step), :724-752
 #pragma unroll
         for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fa[k] / 2;
-        ss_append(g, u);
+        SS_APPEND(g);
 #pragma unroll
         for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fb[k] / 2;
-        ss_append(g, u);
+        SS_APPEND(g);
 #endif
 #endif
         tf += dt;
     }
+#if SS_PREC == 1
+    if (count > 0) ss_mixed_to_u(gsum, bres, u);
+#endif
+#undef SS_APPEND
 }
 
 // Start time of coarse step i, time_end_points[0] + time_step_output*i (:818),

@@ -47,7 +47,7 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 1;
+constexpr int kAbiVersion = 2;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
 constexpr int kScanChunk = 8;   // matrices per lane in the state scan
 constexpr int kScanBlock = 256;
@@ -364,7 +364,7 @@ int default_lanes(int64_t n_coarse, int n_fine)
 }
 
 std::string build_source(const char* field_src, int dim, int method, int expo, int rotating,
-                         int number_of_squares, int field_has_context)
+                         int number_of_squares, int field_has_context, int precision)
 {
     int hyper = (number_of_squares + 1) / 2;  // ceil(squares / 2), :1547
     if (number_of_squares <= 0) hyper = 0;
@@ -372,8 +372,9 @@ std::string build_source(const char* field_src, int dim, int method, int expo, i
     char head[512];
     snprintf(head, sizeof(head),
              "#define SS_DIM %d\n#define SS_LIE %d\n#define SS_METHOD %d\n#define SS_EXPO %d\n"
-             "#define SS_ROT %d\n#define SS_HYPER %d\n#define SS_CTX %d\n",
-             dim, dim == 2 ? 3 : 4, method, expo, rotating ? 1 : 0, hyper, field_has_context ? 1 : 0);
+             "#define SS_ROT %d\n#define SS_HYPER %d\n#define SS_CTX %d\n#define SS_PREC %d\n",
+             dim, dim == 2 ? 3 : 4, method, expo, rotating ? 1 : 0, hyper, field_has_context ? 1 : 0,
+             precision);
     s += head;
     s += ss_src_args;
     s += ss_src_complex;
@@ -409,8 +410,8 @@ uint64_t ss_source_hash(void)
 void ss_free(void* p) { std::free(p); }
 
 int ss_compile_code(const char* field_src, int dim, int method, int expo, int rotating,
-                    int number_of_squares, int field_has_context, void** code, size_t* code_size,
-                    char** log)
+                    int number_of_squares, int field_has_context, int precision, void** code,
+                    size_t* code_size, char** log)
 {
     if (code) *code = nullptr;
     if (code_size) *code_size = 0;
@@ -421,8 +422,10 @@ int ss_compile_code(const char* field_src, int dim, int method, int expo, int ro
     if (expo != 0 && expo != 1) return fail("ss_compile_code: bad exponentiation method %d", expo);
     if (expo == 0 && dim != 2) return fail("ss_compile_code: analytic exponentiation is spin-half only");
     if (number_of_squares > 60) return fail("ss_compile_code: number_of_squares %d too large", number_of_squares);
+    if (precision != SS_PRECISION_FP64 && precision != SS_PRECISION_FP32_BULK)
+        return fail("ss_compile_code: bad precision %d", precision);
     const std::string src = build_source(field_src, dim, method, expo, rotating, number_of_squares,
-                                         field_has_context);
+                                         field_has_context, precision);
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "spinsim_field.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
@@ -481,13 +484,13 @@ int ss_module_load(const void* code, size_t code_size, int dim, int method, int 
 }
 
 int ss_compile(const char* field_src, int dim, int method, int expo, int rotating, int number_of_squares,
-               int field_has_context, ss_module** out)
+               int field_has_context, int precision, ss_module** out)
 {
     void* code = nullptr;
     size_t size = 0;
     char* log = nullptr;
     int rc = ss_compile_code(field_src, dim, method, expo, rotating, number_of_squares, field_has_context,
-                             &code, &size, &log);
+                             precision, &code, &size, &log);
     std::free(log);
     if (rc) return rc;
     rc = ss_module_load(code, size, dim, method, expo, rotating, out);

@@ -36,12 +36,16 @@ def _cache_dir():
     return d
 
 
+PRECISIONS = {"fp64": 0, "fp32_bulk": 1}
+
+
 def compile_field(src: str, dim: int, method: int, expo: int, rotating: bool,
-                  number_of_squares: int, has_context: bool) -> bytes:
+                  number_of_squares: int, has_context: bool, precision: int = 0) -> bytes:
     """hipRTC code object for one field source and method set (memory and disk cached)."""
     L = _native.lib()
     key_text = "\x00".join(map(str, (src, dim, method, expo, int(rotating), number_of_squares,
-                                      int(has_context), L.ss_source_hash(), L.ss_abi_version())))
+                                      int(has_context), precision, L.ss_source_hash(),
+                                      L.ss_abi_version())))
     key = hashlib.sha256(key_text.encode()).hexdigest()[:32]
     with _code_lock:
         if key in _code_cache:
@@ -53,7 +57,7 @@ def compile_field(src: str, dim: int, method: int, expo: int, rotating: bool,
             code = fh.read()
     if not code:
         code, _log = _native.compile_code(src, dim, method, expo, rotating, number_of_squares,
-                                          has_context)
+                                          has_context, precision)
         try:
             os.makedirs(os.path.dirname(path), exist_ok=True)
             tmp = f"{path}.tmp{os.getpid()}"
@@ -179,8 +183,16 @@ def derive_steps(time_start, time_end, time_step_integration, time_step_output, 
 class Simulator:
     """Drop-in for spinsim.Simulator (spinsim/__init__.py:334-1111).
 
-    Extra keyword arguments: ``samplers`` (for HIP-C field text) and
-    ``lanes_per_step`` (lanes sharing one coarse step; default: tuned).
+    Extra keyword arguments: ``samplers`` (for HIP-C field text),
+    ``lanes_per_step`` (lanes sharing one coarse step; default: tuned) and
+    ``precision``: ``"fp64"`` (default; the reference's arithmetic) or
+    ``"fp32_bulk"`` -- the higher-order residual exp(A) - I - A of each step
+    operator and the products of residuals within a lane's chunk of steps in
+    float32 (a scaled Taylor series instead of the exponentiation method),
+    while field samples, the frame, first-order terms, the products across
+    lanes, the outputs and the state scan stay float64.  Accurate when each
+    step's rotation angle is small (rotating frame); check with
+    :func:`spinsim_amd.precision_error`.
     ``threads_per_block``, ``max_registers`` and ``number_of_threads`` are
     accepted for compatibility and do not change the gfx950 launch.
     """
@@ -189,7 +201,8 @@ class Simulator:
                  exponentiation_method: ExponentiationMethod = None, use_rotating_frame: bool = True,
                  integration_method: IntegrationMethod = IntegrationMethod.MAGNUS_CF4,
                  number_of_squares: int = 24, threads_per_block: int = 64, max_registers: int = None,
-                 number_of_threads: int = None, *, samplers=None, lanes_per_step: int = None):
+                 number_of_threads: int = None, *, samplers=None, lanes_per_step: int = None,
+                 precision: str = "fp64"):
         if not device:
             device = _default_device()
         if device != Device.ROCM:
@@ -204,6 +217,9 @@ class Simulator:
         self.integration_method = integration_method
         self.number_of_squares = int(number_of_squares)
         self.lanes_per_step = lanes_per_step
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
         self.exponentiation_method = resolve_exponentiation(spin_quantum_number,
                                                             exponentiation_method)
         self.dimension = spin_quantum_number.dimension
@@ -214,7 +230,7 @@ class Simulator:
             self._code = compile_field(self.field.source, self.dimension,
                                        integration_method.index, self.exponentiation_method.index,
                                        self.use_rotating_frame, self.number_of_squares,
-                                       self.field.has_context)
+                                       self.field.has_context, PRECISIONS[precision])
         except Exception:
             print(f"{_RED}spinsim error!!!\nhipRTC could not compile get_field() into a device "
                   f"function.{_RESET}\n")
