@@ -1,13 +1,20 @@
 """Benchmark: Magnus-CF4 steps/s of spinsim_amd's integration path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-variants] [--copy-probe MB]
 
-A step is one evaluate() pass of the spin-one Rabi workload (x = 2pi*2 kHz
+Headline workload (``value``): the spin-one Rabi drive -- x = 2pi*2 kHz
 cos(2pi*700 kHz t), z = 2pi*700 kHz, quadratic shift 2pi*1 kHz; t = 0 -> 100 ms,
-integration step 100 ns, output step 1 us: 1e5 coarse x 10 fine = 1e6 CF4
-steps; Lie-Trotter exponentials, rotating frame, float64) over inputs already
-resident in HBM: the time-evolution kernel, the state prefix product and the
-Bloch vectors.  Prints one JSON line.
+integration step 100 ns, output step 1 us: 1e5 coarse x 10 fine = 1e6 CF4 steps;
+Lie-Trotter exponentials (24 squares), rotating frame, plain float64.  A step
+is one evaluate() pass over inputs resident in HBM: the time-evolution kernel,
+the state prefix product and the Bloch vectors (outputs stay on the device).
+
+Also reported (``variants``): the same workload with float32-bulk step
+operators, and the batched ensemble of 4096 spin-half Rabi simulations with
+uneven durations (1-50 ms, 50 ns steps, 1 us outputs; float32-bulk; one
+persistent launch + per-simulation state scans replayed as a HIP graph),
+each with its state error against the float64 oracle.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -29,6 +36,12 @@ __device__ void get_field(double t, const double* s, double* f)
 { f[0] = SS_TAU * 2e3 * cos(SS_TAU * 7e5 * t); f[1] = 0; f[2] = SS_TAU * 7e5; f[3] = SS_TAU * 1e3; }
 """
 T0, T1, DT, DT_OUT = 0.0, 100e-3, 100e-9, 1e-6
+
+# Swept spin-half Rabi drive: sweep = (carrier, Rabi amplitude, detuning), rad/s.
+RABI_SWEEP = """
+__device__ void get_field(double t, const double* s, double* f)
+{ f[0] = 2 * s[1] * cos(s[0] * t); f[1] = 0; f[2] = s[0] + s[2]; }
+"""
 
 # Algorithmic float64 flops per spin-one Lie-Trotter CF4 fine step, counted from
 # the reference's expressions (1 flop per real +,-,*,/,sqrt; complex * = 6,
@@ -52,34 +65,15 @@ def cpu_baseline(co, psi0, dt, n, nf, cores, reps=3):
     return statistics.median(rates), state
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--lanes", type=int, default=None)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the integration path "
-              "is data-parallel over coarse steps and simulations and is benchmarked on one "
-              "device", file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-    import torch
-
-    import spinsim_amd as ss
+def single_workload(ss, torch, precision, steps, warmup, lanes):
+    """Time K evaluate() passes of the spin-one workload; returns a dict and the state."""
     from spinsim_amd.simulator import derive_steps
 
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    sim = ss.Simulator(RABI_ONE, ss.SpinQuantumNumber.ONE)
+    sim = ss.Simulator(RABI_ONE, ss.SpinQuantumNumber.ONE, precision=precision)
     dt, n, _ = derive_steps(T0, T1, DT, DT_OUT)
     nf = math.floor(DT_OUT / dt + 0.5)
-    fine_steps = n * nf
     psi0 = ss.SpinQuantumNumber.ONE.plus_z
-
     sweep = torch.zeros(1, dtype=torch.float64, device=dev)
     time_out = torch.empty(n, dtype=torch.float64, device=dev)
     u = torch.empty((n, 3, 3), dtype=torch.complex128, device=dev)
@@ -87,34 +81,120 @@ def main():
     spin = torch.empty((n, 3), dtype=torch.float64, device=dev)
     ws = sim.scan_workspace(n, dev)
     stream = torch.cuda.current_stream(dev)
+    if lanes == "auto":
+        from spinsim_amd.autotune import tune_lanes
+        lanes, _ = tune_lanes(sim, n, nf, dt, DT_OUT, sweep, T0)
 
     def step(events=None):
         if events is not None:
             events[0].record(stream)
-        sim.launch_time_evolution(sweep, T0, dt, DT_OUT, time_out, u, args.lanes)
+        sim.launch_time_evolution(sweep, T0, dt, DT_OUT, time_out, u, lanes)
         if events is not None:
             events[1].record(stream)
         sim.launch_state_scan(psi0, u, state, ws)
         sim.launch_spin(state, spin)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
+          for _ in range(steps)]
     t_start = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         step(ev[k])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     kernel_ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
+    out = {"steps_per_s": n * nf / (elapsed / steps), "ms_per_step": 1e3 * elapsed / steps,
+           "kernel_ms": kernel_ms, "lanes_per_step": lanes, "coarse": n, "fine": n * nf,
+           "dt": dt, "n_fine": nf}
+    return out, state.cpu().numpy(), psi0
 
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = fine_steps / (elapsed / args.steps)
-    achieved = fine_steps * FLOPS_PER_STEP / (kernel_ms * 1e-3) / 1e12
+
+def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
+    """4096 spin-half Rabi simulations with uneven durations in one persistent launch."""
+    import numpy as np
+
+    from oracle import spinsim_oracle as O
+    from oracle.c_oracle import COracle
+    from spinsim_amd.autotune import tune_batch
+
+    rng = np.random.default_rng(20261202)
+    tau = 2 * math.pi
+    sweep = np.stack([np.full(n_sims, tau * 7e5), tau * rng.uniform(500, 5000, n_sims),
+                      tau * rng.normal(0, 200, n_sims)], 1)
+    dur = rng.uniform(1e-3, 50e-3, n_sims)
+    sim = ss.Simulator(RABI_SWEEP, ss.SpinQuantumNumber.HALF, precision="fp32_bulk")
+    psi0 = ss.SpinQuantumNumber.HALF.plus_z
+    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep)
+    tune_batch(plan, reps=1, warmup=0, lanes=(1, 2, 4), items=(1, 4, 16))
+    plan.capture()
+    for _ in range(warmup):
+        plan.replay()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        plan.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    errs = []
+    co = COracle(RABI_SWEEP, 2)
+    for s in np.linspace(0, n_sims - 1, check).astype(int):
+        dt, n, nf, _ = O.timing(0.0, dur[s], 50e-9, 1e-6)
+        _, u = co.time_evolution(sweep[s], 0.0, dt, 1e-6, nf, n)
+        ref = co.get_state(psi0, u)
+        a, b = int(plan.offsets_host[s]), int(plan.offsets_host[s + 1])
+        errs.append(float(np.abs(plan.state[a:b].cpu().numpy() - ref).max()))
+    return {"steps_per_s": plan.fine_steps / (elapsed / steps), "ms_per_step": 1e3 * elapsed / steps,
+            "fine_steps": plan.fine_steps, "coarse_rows": plan.total, "simulations": n_sims,
+            "lanes_per_step": plan.lanes, "items_per_grab": plan.items_per_grab,
+            "state_err_vs_oracle_sampled": max(errs), "sampled_simulations": check,
+            "dtype": "f32 residuals / f64 samples, products, scan", "graph": True}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--lanes", default="auto")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run one device copy of this many MiB (PMC calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
+              "integration path on a single device", file=sys.stderr)
+        sys.exit(2)
+    lanes = args.lanes if args.lanes == "auto" else int(args.lanes)
+
+    import numpy as np
+    import torch
+
+    import spinsim_amd as ss
+    from oracle.c_oracle import COracle
+
+    torch.cuda.set_device(0)
+    main_run, state, psi0 = single_workload(ss, torch, "fp64", args.steps, args.warmup, lanes)
+    n, nf, dt = main_run["coarse"], main_run["n_fine"], main_run["dt"]
+    achieved = main_run["fine"] * FLOPS_PER_STEP / (main_run["kernel_ms"] * 1e-3) / 1e12
+
+    variants = {}
+    if not args.no_variants:
+        v32, state32, _ = single_workload(ss, torch, "fp32_bulk", args.steps, args.warmup, lanes)
+        variants["fp32_bulk"] = {k: v32[k] for k in ("steps_per_s", "ms_per_step", "kernel_ms",
+                                                      "lanes_per_step")}
+        variants["batched_4096_spin_half"] = batched_workload(ss, torch, max(3, args.steps // 4),
+                                                              1)
+
+    if args.copy_probe:
+        src = torch.empty(args.copy_probe << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     # Accuracy against the float64 oracle (C restatement) on the same inputs.
-    from oracle.c_oracle import COracle
     co = COracle(RABI_ONE, 3)
     cores = min(len(os.sched_getaffinity(0)), 16)
     base = None
@@ -124,22 +204,27 @@ def main():
     else:
         rate, ref_state = cpu_baseline(co, psi0, dt, n, nf, cores)
         base = {"value": rate, "unit": "steps/s", "cores": cores, "kind": "port",
-                "sample": f"whole workload ({fine_steps} CF4 steps: time evolution, state "
+                "sample": f"whole workload ({n * nf} CF4 steps: time evolution, state "
                           f"product, spin) on {cores} host threads, median of 3"}
-    state_err = float(np.abs(state.cpu().numpy() - ref_state).max())
+    state_err = float(np.abs(state - ref_state).max())
+    if "fp32_bulk" in variants:
+        variants["fp32_bulk"]["state_err_vs_oracle"] = float(np.abs(state32 - ref_state).max())
 
     line = {
-        "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "metric": METRIC, "value": main_run["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_run["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
         "config": {"workload": "spin-one Rabi drive, Lie-Trotter (24 squares), rotating frame, "
                                "plain float64, t 0-100 ms, dt 100 ns, output 1 us",
-                   "coarse_steps": n, "fine_steps": fine_steps, "parallelism": "single GPU"},
+                   "coarse_steps": n, "fine_steps": n * nf,
+                   "lanes_per_step": main_run["lanes_per_step"], "parallelism": "single GPU"},
         "state_err_vs_oracle": state_err,
-        "time_evolution_kernel_ms": kernel_ms,
+        "time_evolution_kernel_ms": main_run["kernel_ms"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
         "cpu_baseline": base,
+        "variants": variants,
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,137 @@
+"""Launch-shape autotuner for the time-evolution kernels.
+
+Chooses how many lanes share one coarse step (and, for batches, how many
+rows a wave takes per grab of the work counter) by timing the candidates on
+the device with HIP events.  Results are cached per (kernel, shape, device)
+in memory and in a JSON file next to the code-object cache.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import statistics
+import threading
+
+_lock = threading.Lock()
+_memo: dict = {}
+
+
+def lane_candidates(n_fine: int, max_lanes: int = 64):
+    """Divisors of n_fine (balanced chunks) and powers of two, up to 64."""
+    top = max(1, min(int(n_fine), max_lanes))
+    c = {g for g in range(1, top + 1) if n_fine % g == 0}
+    g = 1
+    while g <= top:
+        c.add(g)
+        g *= 2
+    return sorted(c)
+
+
+def _cache_path():
+    from .simulator import _cache_dir
+    return os.path.join(_cache_dir(), "autotune.json")
+
+
+def _load_disk():
+    try:
+        with open(_cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_disk(key, value):
+    try:
+        data = _load_disk()
+        data[key] = value
+        os.makedirs(os.path.dirname(_cache_path()), exist_ok=True)
+        tmp = _cache_path() + f".tmp{os.getpid()}"
+        with open(tmp, "w") as fh:
+            json.dump(data, fh)
+        os.replace(tmp, _cache_path())
+    except OSError:
+        pass
+
+
+def _key(sim, *shape):
+    import torch
+
+    h = hashlib.sha256(sim._code).hexdigest()[:16]
+    name = torch.cuda.get_device_name(torch.cuda.current_device())
+    return "|".join([h, name, *map(str, shape)])
+
+
+def time_launch(fn, reps=3, warmup=1):
+    """Median milliseconds of fn() on the current stream (HIP events)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    times = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    return statistics.median(times)
+
+
+def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep, t0: float = 0.0,
+               reps: int = 3, use_cache: bool = True):
+    """Best lanes_per_step for one simulation of this shape; returns (lanes, table)."""
+    import torch
+
+    key = _key(sim, "single", n_coarse, n_fine)
+    if use_cache:
+        with _lock:
+            if key in _memo:
+                return _memo[key], None
+        disk = _load_disk()
+        if key in disk:
+            with _lock:
+                _memo[key] = int(disk[key])
+            return int(disk[key]), None
+    dev = torch.device("cuda", torch.cuda.current_device())
+    d = sim.dimension
+    time_out = torch.empty(n_coarse, dtype=torch.float64, device=dev)
+    u = torch.empty((n_coarse, d, d), dtype=torch.complex128, device=dev)
+    sw = sweep if isinstance(sweep, torch.Tensor) else torch.as_tensor(sweep, dtype=torch.float64,
+                                                                        device=dev)
+    table = {}
+    for g in lane_candidates(n_fine):
+        table[g] = time_launch(
+            lambda: sim.launch_time_evolution(sw, t0, dt, dt_out, time_out, u, g), reps=reps)
+    best = min(table, key=table.get)
+    with _lock:
+        _memo[key] = best
+    _store_disk(key, best)
+    return best, table
+
+
+def tune_batch(plan, reps: int = 2, use_cache: bool = True, lanes=None, items=(1, 4, 16),
+               warmup: int = 1):
+    """Best (lanes_per_step, items_per_grab) for a BatchPlan; applied to the plan.
+
+    ``lanes`` defaults to :func:`lane_candidates` of the largest n_fine.
+    """
+    nf_max = int(plan.n_fine.max().item()) if plan.total else 1
+    lanes = tuple(lanes) if lanes is not None else tuple(lane_candidates(nf_max))
+    key = _key(plan.sim, "batch", plan.n_sims, plan.total, nf_max, lanes, tuple(items))
+    if use_cache:
+        disk = _load_disk()
+        if key in disk:
+            plan.lanes, plan.items_per_grab = map(int, disk[key])
+            return (plan.lanes, plan.items_per_grab), None
+    table = {}
+    for g in lanes:
+        for it in items:
+            plan.lanes, plan.items_per_grab = g, it
+            table[(g, it)] = time_launch(plan.run_time_evolution, reps=reps, warmup=warmup)
+    best = min(table, key=table.get)
+    plan.lanes, plan.items_per_grab = best
+    _store_disk(key, list(best))
+    return best, table

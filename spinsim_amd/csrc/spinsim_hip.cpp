@@ -356,11 +356,20 @@ ScanLayout scan_layout(int64_t n, int dim)
 
 int default_lanes(int64_t n_coarse, int n_fine)
 {
-    // Aim for about eight waves per SIMD of an MI355X (256 CUs x 4 SIMDs).
+    // The smallest divisor of n_fine (equal chunks per lane) that gives about
+    // eight waves per SIMD of an MI355X (256 CUs x 4 SIMDs); else the largest
+    // divisor up to 64, or a power of two when n_fine has no useful divisor.
     const long long target = 256LL * 4 * 8 * 64;
-    int g = 1;
-    while (g < 64 && 2 * g <= n_fine && n_coarse * g < target) g *= 2;
-    return g;
+    int best = 1;
+    for (int g = 1; g <= 64 && g <= n_fine; ++g) {
+        if (n_fine % g) continue;
+        best = g;
+        if (n_coarse * g >= target) return g;
+    }
+    if (n_coarse * best * 4 >= target) return best;
+    int p = 1;
+    while (p < 64 && 2 * p <= n_fine && n_coarse * p < target) p *= 2;
+    return std::max(p, best);
 }
 
 std::string build_source(const char* field_src, int dim, int method, int expo, int rotating,

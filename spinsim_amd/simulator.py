@@ -184,7 +184,9 @@ class Simulator:
     """Drop-in for spinsim.Simulator (spinsim/__init__.py:334-1111).
 
     Extra keyword arguments: ``samplers`` (for HIP-C field text),
-    ``lanes_per_step`` (lanes sharing one coarse step; default: tuned) and
+    ``lanes_per_step`` (lanes sharing one coarse step: an int in 1..64,
+    ``"auto"`` to time the candidates on first use of each shape, or None for
+    the library's divisor-of-n_fine default) and
     ``precision``: ``"fp64"`` (default; the reference's arithmetic) or
     ``"fp32_bulk"`` -- the higher-order residual exp(A) - I - A of each step
     operator and the products of residuals within a lane's chunk of steps in
@@ -279,6 +281,10 @@ class Simulator:
         dev = u_out.device
         n = int(time_out.shape[0])
         lanes = lanes_per_step if lanes_per_step is not None else (self.lanes_per_step or 0)
+        if lanes == "auto":
+            from .autotune import tune_lanes
+            n_fine = math.floor(dt_out / dt + 0.5)
+            lanes, _ = tune_lanes(self, n, n_fine, dt, dt_out, sweep, t0) if n else (0, None)
         if stream is None:
             stream = torch.cuda.current_stream(dev).cuda_stream
         mod = self._module(dev)
