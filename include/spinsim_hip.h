@@ -53,6 +53,17 @@ typedef struct ss_table {
     int32_t zero_boundary; /* 1: zero outside [time[0], time[-1]] */
 } ss_table;
 
+/* One differentiated control of the adjoint: field[axis] += scale *
+ * (interpolation table `table`)(t); its gradient occupies entries
+ * [grad_offset, grad_offset + table size) of each simulation's gradient row.
+ * (No reference counterpart: the reference has no gradients.) */
+typedef struct ss_control_desc {
+    int32_t axis;  /* 0 x, 1 y, 2 z, 3 quadratic shift (spin one) */
+    int32_t table;
+    double scale;
+    int64_t grad_offset;
+} ss_control_desc;
+
 enum { /* IntegrationMethod, :68-93 */
     SS_METHOD_MAGNUS_CF4 = 0,
     SS_METHOD_MIDPOINT_SAMPLE = 1, /* the reference's EULER */
@@ -134,6 +145,30 @@ int ss_state_scan(const double *psi0, const double *u, double *state, int64_t n,
 /* Per-simulation state scan over CSR rows; psi0: device (n_sims, dim). */
 int ss_batch_state_scan(const double *psi0, const double *u, double *state,
                         const int64_t *offsets, int64_t n_sims, int dim, ss_stream stream);
+
+/* Costates for the adjoint: costate[o+n-1] = lambda_final[s] and
+ * costate[i] = u[i+1]^+ costate[i+1] over each simulation's rows, i.e. the
+ * costate at the END of every coarse interval.  lambda_final: device (n_sims,
+ * dim) complex, the loss gradient with respect to psi(T) (2 dL/dpsi*). */
+int ss_batch_costate_scan(const double *lambda_final, const double *u, double *costate,
+                          const int64_t *offsets, int64_t n_sims, int dim, ss_stream stream);
+/* psi(T) of each simulation: u[last] state[last] (psi0 for a simulation with
+ * no rows); out: device (n_sims, dim) complex. */
+int ss_batch_final_state(const double *psi0, const double *u, const double *state,
+                         const int64_t *offsets, double *out, int64_t n_sims, int dim,
+                         ss_stream stream);
+/* Adjoint gradient of a real loss of psi(T) with respect to the control
+ * amplitudes, for a module whose field reads the controls' tables.  Needs the
+ * forward batch's u and state and the costates; recomputes the fine steps.
+ * grad (n_sims, grad_stride) and *status are cleared by the call; status
+ * becomes 1 if a step's generator was too large for the derivative series.
+ * (No reference counterpart: the reference has no gradients.) */
+int ss_batch_adjoint(const ss_module *m, const double *sweep, int64_t sweep_stride, int64_t n_sims,
+                     const int64_t *offsets, const double *t0, const double *dt,
+                     const double *dt_out, const int32_t *n_fine, int64_t total, const double *u,
+                     const double *state, const double *costate, const ss_table *tables,
+                     const ss_control_desc *controls, int n_controls, double *grad,
+                     int64_t grad_stride, int32_t *status, ss_stream stream);
 
 /* Evaluate interpolation table k (row `sim` when batched) at n device
  * times: the sampler of generate_interpolation_sampler, :2065-2142. */

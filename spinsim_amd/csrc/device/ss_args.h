@@ -44,3 +44,40 @@ struct ss_batch_args {
     int lanes_per_step;
     int items_per_grab;
 };
+
+// One control: field[axis] += scale * (interpolation table `table`)(t), whose
+// amplitudes are the differentiated parameters.
+struct ss_control {
+    int axis;                    // 0..3: x, y, z, q
+    int table;                   // slot in the ss_table array
+    double scale;
+    long long grad_offset;       // offset of this control's block in a gradient row
+};
+
+#define SS_MAX_CONTROLS 4
+
+// Batched adjoint launch: one lane per coarse row.  state[row] is the state at
+// the start of the row's interval, costate[row] the costate at its end.
+struct ss_adjoint_args {
+    const double* sweep;
+    const void* tables;
+    const long long* offsets;
+    const double* t0;
+    const double* dt;
+    const double* dt_out;
+    const int* n_fine;
+    const double* u;             // (total, d, d)
+    const double* state;         // (total, d)
+    const double* costate;       // (total, d)
+    double* grad;                // (B, grad_stride), accumulated with atomics
+    int* status;                 // set to 1 if a step is too large for the series
+    long long total;
+    long long n_sims;
+    long long sweep_stride;
+    long long grad_stride;
+    double off_a_frac;
+    double off_b_frac;
+    int n_controls;
+    int pad_;
+    ss_control controls[SS_MAX_CONTROLS];
+};

@@ -677,3 +677,400 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
         }
     }
 }
+
+// ===========================================================================
+// Adjoint: gradient of a real loss of the final states with respect to the
+// amplitudes of interpolated control fields.
+//
+// For a step psi_{j+1} = E_j psi_j with E_j = exp(A_j), A_j = -i G(g_j), and
+// costate lambda = dL/dpsi (torch's convention, 2 dL/dpsi*), propagated
+// backward as lambda_j = E_j^+ lambda_{j+1}:
+//     dL/dg_k = Re( lambda_{j+1}^+ L(A_j, A(e_k)) psi_j ),
+// L the Frechet derivative of exp, from its series
+//     L(A, E) = sum_n 1/(n+1)! sum_{p+q=n} A^p E A^q .
+// Within a coarse interval the state and costate are both walked backward
+// from the interval's end (the steps are unitary), so only the coarse
+// operators and states are stored; the fine steps are recomputed.
+// ===========================================================================
+#define SS_ADJ_ORDER 6       // Frechet series order (terms up to n = 6)
+#define SS_ADJ_TAYLOR 9      // exp(-A) v Taylor order
+#define SS_ADJ_NU 0.05       // sub-step until the generator norm bound is below this
+#define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 51)
+#define SS_ADJ_WINDOW 8      // gradient taps kept per lane and control in LDS
+
+struct ss_gen64 {
+    double d[SS_D];
+    ss_cd u, l;
+};
+
+// A = -i G(g) scaled: imaginary diagonal (0, d), A[k][k+1] = u, A[k+1][k] = l.
+static __device__ __forceinline__ ss_gen64 ss_generator64(const double* g, double scale)
+{
+    ss_gen64 a;
+#if SS_DIM == 2
+    const double hx = 0.5 * scale * g[0], hy = 0.5 * scale * g[1], hz = 0.5 * scale * g[2];
+    a.d[0] = -hz;
+    a.d[1] = hz;
+    a.u = ss_c(-hy, -hx);
+    a.l = ss_c(hy, -hx);
+#else
+    const double rx = scale * g[0] * SS_INV_SQRT2, ry = scale * g[1] * SS_INV_SQRT2;
+    const double z = scale * g[2], q3 = scale * g[3] * (1.0 / 3.0);
+    a.d[0] = -(z + q3);
+    a.d[1] = 2.0 * q3;
+    a.d[2] = z - q3;
+    a.u = ss_c(-ry, -rx);
+    a.l = ss_c(ry, -rx);
+#endif
+    return a;
+}
+
+// out = s A v  (s = +-1 or a scale)
+static __device__ __forceinline__ void ss_gen_vec(const ss_gen64& a, double s, const ss_cd v[SS_D], ss_cd out[SS_D])
+{
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r) {
+        ss_cd acc = ss_c(-a.d[r] * v[r].y, a.d[r] * v[r].x);
+        if (r > 0) acc = ss_mac(acc, a.l, v[r - 1]);
+        if (r + 1 < SS_D) acc = ss_mac(acc, a.u, v[r + 1]);
+        out[r] = ss_scale(acc, s);
+    }
+}
+
+// v <- exp(-A) v by its Taylor series (|A| <= SS_ADJ_NU: truncation < 1e-17).
+static __device__ __forceinline__ void ss_expm_neg_vec(const ss_gen64& a, ss_cd v[SS_D])
+{
+    ss_cd term[SS_D], acc[SS_D];
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r) {
+        term[r] = v[r];
+        acc[r] = v[r];
+    }
+#pragma unroll
+    for (int n = 1; n <= SS_ADJ_TAYLOR; ++n) {
+        ss_cd t[SS_D];
+        ss_gen_vec(a, -1.0 / n, term, t);
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) {
+            term[r] = t[r];
+            acc[r] = ss_add(acc[r], t[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r) v[r] = acc[r];
+}
+
+static __device__ __forceinline__ double ss_re_dot(const ss_cd a[SS_D], const ss_cd b[SS_D])
+{
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r) s += a[r].x * b[r].x + a[r].y * b[r].y;  // Re(a^+ b)
+    return s;
+}
+
+// Accumulate grad[k] += w * Re(lam^+ L(A, A(e_k)) psi) for the axes in mask.
+static __device__ __forceinline__ void ss_frechet_grad(const ss_gen64& a, const ss_cd lam[SS_D],
+                                                       const ss_cd psi[SS_D], unsigned mask, double w,
+                                                       double grad[SS_LIE])
+{
+    // beta_q = A^q psi
+    ss_cd beta[SS_ADJ_ORDER + 1][SS_D];
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r) beta[0][r] = psi[r];
+#pragma unroll
+    for (int q = 1; q <= SS_ADJ_ORDER; ++q) ss_gen_vec(a, 1.0, beta[q - 1], beta[q]);
+    // gamma_q = sum_p c_{p+q} (-A)^p lam, c_n = 1/(n+1)!, by Horner
+    const double c[SS_ADJ_ORDER + 1] = {1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+#pragma unroll
+    for (int q = 0; q <= SS_ADJ_ORDER; ++q) {
+        ss_cd gam[SS_D];
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) gam[r] = ss_scale(lam[r], c[SS_ADJ_ORDER]);
+#pragma unroll
+        for (int p = SS_ADJ_ORDER - q - 1; p >= 0; --p) {
+            ss_cd t[SS_D];
+            ss_gen_vec(a, -1.0, gam, t);
+#pragma unroll
+            for (int r = 0; r < SS_D; ++r) gam[r] = ss_add(t[r], ss_scale(lam[r], c[p + q]));
+        }
+#pragma unroll
+        for (int k = 0; k < SS_LIE; ++k) {
+            if (!(mask & (1u << k))) continue;
+            double e[4] = {0.0, 0.0, 0.0, 0.0};
+            e[k] = 1.0;
+            const ss_gen64 ek = ss_generator64(e, 1.0);
+            ss_cd t[SS_D];
+            ss_gen_vec(ek, 1.0, beta[q], t);
+            grad[k] += w * ss_re_dot(gam, t);
+        }
+    }
+}
+
+// One exponential walked backward: psi <- exp(-A) psi, lambda <- exp(-A) lambda,
+// grad += dL/dg.  Large steps are split into 2^s equal sub-steps.
+static __device__ __forceinline__ void ss_adjoint_exp(const double g[SS_FIELD_LEN], ss_cd psi[SS_D], ss_cd lam[SS_D],
+                                                      unsigned mask, double grad[SS_LIE], int* status)
+{
+#if SS_DIM == 2
+    const double nu = 0.5 * (fabs(g[0]) + fabs(g[1]) + fabs(g[2]));
+#else
+    const double nu = fabs(g[0]) + fabs(g[1]) + fabs(g[2]) + fabs(g[3]);
+#endif
+    int split = 0;
+    while (nu * ldexp(1.0, -split) > SS_ADJ_NU && split < SS_ADJ_MAX_SPLIT) ++split;
+    if (nu * ldexp(1.0, -split) > SS_ADJ_NU) *status = 1;
+    const double scale = ldexp(1.0, -split);
+    const ss_gen64 a = ss_generator64(g, scale);
+    const int pieces = 1 << split;
+#pragma unroll 1
+    for (int r = 0; r < pieces; ++r) {
+        ss_expm_neg_vec(a, psi);                         // state before this piece
+        ss_frechet_grad(a, lam, psi, mask, scale, grad);  // d/dg of the piece = scale * d/dg_s
+        ss_expm_neg_vec(a, lam);
+    }
+}
+
+// Interpolation weights of a table at t: value = sum_k w[k] amp[first + k].
+static __device__ __forceinline__ void ss_interp_weights(const ss_table* tb, double t, long long* first, double w[4])
+{
+    const double* time = tb->time;
+    const long long size = tb->size;
+    w[0] = w[1] = w[2] = w[3] = 0.0;
+    *first = 0;
+    if (!tb->zero_boundary) {
+        if (t <= time[0]) { w[0] = 1.0; return; }
+        if (t >= time[size - 1]) { *first = size - 1; w[0] = 1.0; return; }
+    }
+    double p;
+    const long long i = ss_find_time_index(tb, t, &p);
+    if (tb->order == 0) {
+        if (i >= 0) { *first = i; w[0] = 1.0; }
+        return;
+    }
+    if (!(i >= 0 && i + 1 < size)) return;
+    if (tb->order == 1) {
+        *first = i;
+        w[0] = 1 - p;
+        w[1] = p;
+        return;
+    }
+    // cubic Hermite: value = a_i h00 + a_{i+1} h01 + h L h10 + h R h11, with the
+    // tangents L, R linear in a_{i-1} .. a_{i+2} (:2108-2141)
+    const double h = time[i + 1] - time[i];
+    const double dm0 = (time[i] != time[i + 1]) ? -1.0 / h : 0.0;   // d mid / d a_i
+    const double dm1 = -dm0;                                        // d mid / d a_{i+1}
+    double l_m1 = 0, l_0 = 0, l_1 = 0, r_0 = 0, r_1 = 0, r_2 = 0;
+    if (i == 0) {
+        l_0 = dm0;
+        l_1 = dm1;
+    } else if (time[i - 1] != time[i]) {
+        const double hl = time[i] - time[i - 1];
+        l_m1 = -0.5 / hl;
+        l_0 = 0.5 / hl + 0.5 * dm0;
+        l_1 = 0.5 * dm1;
+    }
+    if (i + 2 == size) {
+        r_0 = dm0;
+        r_1 = dm1;
+    } else if (time[i + 1] != time[i + 2]) {
+        const double hr = time[i + 2] - time[i + 1];
+        r_0 = 0.5 * dm0;
+        r_1 = -0.5 / hr + 0.5 * dm1;
+        r_2 = 0.5 / hr;
+    }
+    const double p2 = p * p, p3 = p2 * p;
+    const double h00 = 2 * p3 - 3 * p2 + 1, h01 = -2 * p3 + 3 * p2;
+    const double h10 = h * (p3 - 2 * p2 + p), h11 = h * (p3 - p2);
+    *first = i - 1;
+    w[0] = h10 * l_m1;
+    w[1] = h00 + h10 * l_0 + h11 * r_0;
+    w[2] = h01 + h10 * l_1 + h11 * r_1;
+    w[3] = h11 * r_2;
+}
+
+// dL/d(lab-frame field) at one sample -> control gradients (LDS window or atomics).
+static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& a, const ss_ctx* ctx, double t,
+                                                         const double dfield[4], double* win,
+                                                         const long long base[SS_MAX_CONTROLS], double* grad_row)
+{
+    for (int c = 0; c < a.n_controls; ++c) {
+        const ss_control ct = a.controls[c];
+        const double v = ct.scale * dfield[ct.axis];
+        if (v == 0.0) continue;
+        long long first;
+        double w[4];
+        ss_interp_weights(&ctx->tables[ct.table], t, &first, w);
+        const long long size = ctx->tables[ct.table].size;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long idx = first + k;
+            if (w[k] == 0.0 || idx < 0 || idx >= size) continue;
+            const long long slot = idx - base[c];
+            if (slot >= 0 && slot < SS_ADJ_WINDOW)
+                win[c * SS_ADJ_WINDOW + slot] += v * w[k];
+            else
+                atomicAdd(grad_row + ct.grad_offset + idx, v * w[k]);
+        }
+    }
+}
+
+// dL/dg of the rotating-frame samples -> lab frame (x, y rotate back; z, q unchanged).
+static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], double rot, double t_rel, double out[4])
+{
+    out[0] = drot[0];
+    out[1] = drot[1];
+    out[2] = drot[2];
+    out[3] = SS_LIE > 3 ? drot[SS_LIE - 1] : 0.0;
+#if SS_ROT
+    const ss_cd w = ss_winding(rot, t_rel);
+#if SS_DIM == 2
+    const ss_cd wf = ss_mul(w, w);
+#else
+    const ss_cd wf = w;
+#endif
+    const ss_cd gw = ss_mul(ss_c(drot[0], drot[1]), wf);
+    out[0] = gw.x;
+    out[1] = gw.y;
+#else
+    (void)rot;
+    (void)t_rel;
+#endif
+}
+
+extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adjoint_args a)
+{
+    __shared__ double windows[256][SS_MAX_CONTROLS * SS_ADJ_WINDOW];
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double* win = windows[threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < SS_MAX_CONTROLS * SS_ADJ_WINDOW; ++k) win[k] = 0.0;
+    if (row >= a.total) return;
+    long long lo = 0, hi = a.n_sims;
+    while (hi - lo > 1) {
+        const long long mid = (lo + hi) >> 1;
+        if (a.offsets[mid] <= row) lo = mid; else hi = mid;
+    }
+    const int sim = (int)lo;
+    const long long local = row - a.offsets[sim];
+    const int n_fine = a.n_fine[sim];
+    const double dt = a.dt[sim], dt_out = a.dt_out[sim];
+    const double* sweep = a.sweep + (long long)sim * a.sweep_stride;
+    ss_ctx ctx;
+    ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
+    ctx.sim = sim;
+    const double tc = ss_coarse_time(a.t0[sim], dt_out, local);
+    const double rot = ss_rotating_wave(tc, dt_out, sweep, &ctx);
+    double* grad_row = a.grad + (long long)sim * a.grad_stride;
+
+    unsigned mask = 0;
+    long long base[SS_MAX_CONTROLS];
+    for (int c = 0; c < a.n_controls; ++c) {
+        const int ax = a.controls[c].axis;
+        mask |= (ax <= 1 && SS_ROT) ? 3u : (1u << (ax == 3 ? SS_LIE - 1 : ax));
+        double p;
+        const long long i0 = ss_find_time_index(&ctx.tables[a.controls[c].table], tc, &p);
+        base[c] = (i0 < 0 ? 0 : i0) - 1;
+    }
+
+    // state and costate at the interval's end, in the rotating frame
+    ss_cd u[SS_D][SS_D], psi0[SS_D], psi[SS_D], lam[SS_D];
+    {
+        const double2* up = reinterpret_cast<const double2*>(a.u) + row * (SS_D * SS_D);
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) u[r][c] = ss_c(up[r * SS_D + c].x, up[r * SS_D + c].y);
+        const double2* sp = reinterpret_cast<const double2*>(a.state) + row * SS_D;
+        const double2* cp = reinterpret_cast<const double2*>(a.costate) + row * SS_D;
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) {
+            psi0[r] = ss_c(sp[r].x, sp[r].y);
+            lam[r] = ss_c(cp[r].x, cp[r].y);
+        }
+    }
+    ss_matvec<SS_D>(u, psi0, psi);
+#if SS_ROT
+    {
+        // U = X P with X = diag(conj(w), 1, w): psi_rot = X^+ psi, lam_rot = X^+ lam
+        double s, c;
+        sincos(rot * dt_out, &s, &c);
+        const ss_cd w = ss_c(c, s);
+        psi[0] = ss_mul(psi[0], w);
+        lam[0] = ss_mul(lam[0], w);
+        psi[SS_D - 1] = ss_mul(psi[SS_D - 1], ss_conj(w));
+        lam[SS_D - 1] = ss_mul(lam[SS_D - 1], ss_conj(w));
+    }
+#endif
+    const double half = 0.5 * dt;
+    const double off_a = half * a.off_a_frac, off_b = half * a.off_b_frac;
+#pragma unroll 1
+    for (int j = n_fine - 1; j >= 0; --j) {
+        double tf = tc;
+#pragma unroll 1
+        for (int k = 0; k < j; ++k) tf += dt;
+        double fa[SS_FIELD_LEN], g1[SS_FIELD_LEN], d1[SS_LIE], dfa[SS_LIE], lab[4];
+        const double tra = (tf + off_a) - tc;
+        const ss_cd wa = ss_winding(rot, tra);
+        ss_call_field(tra + tc, sweep, fa, &ctx);
+        ss_frame(fa, rot, wa);
+#pragma unroll
+        for (int k = 0; k < SS_LIE; ++k) d1[k] = 0.0;
+#if SS_METHOD == 1
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) g1[k] = dt * fa[k];
+        ss_adjoint_exp(g1, psi, lam, mask, d1, a.status);
+#pragma unroll
+        for (int k = 0; k < SS_LIE; ++k) dfa[k] = dt * d1[k];
+        ss_unframe(dfa, rot, tra, lab);
+        ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
+#else
+        double fb[SS_FIELD_LEN], g2[SS_FIELD_LEN], d2[SS_LIE], dfb[SS_LIE];
+        const double trb = (tf + off_b) - tc;
+        const ss_cd wb = ss_winding(rot, trb);
+        ss_call_field(trb + tc, sweep, fb, &ctx);
+        ss_frame(fb, rot, wb);
+#pragma unroll
+        for (int k = 0; k < SS_LIE; ++k) d2[k] = 0.0;
+#if SS_METHOD == 0
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) {
+            g1[k] = dt * (SS_W0 * fa[k] + SS_W1 * fb[k]);
+            g2[k] = dt * (SS_W1 * fa[k] + SS_W0 * fb[k]);
+        }
+        ss_adjoint_exp(g2, psi, lam, mask, d2, a.status);
+        ss_adjoint_exp(g1, psi, lam, mask, d1, a.status);
+#pragma unroll
+        for (int k = 0; k < SS_LIE; ++k) {
+            dfa[k] = dt * (SS_W0 * d1[k] + SS_W1 * d2[k]);
+            dfb[k] = dt * (SS_W1 * d1[k] + SS_W0 * d2[k]);
+        }
+#else
+#pragma unroll
+        for (int k = 0; k < SS_FIELD_LEN; ++k) {
+            g1[k] = dt * fa[k] / 2;
+            g2[k] = dt * fb[k] / 2;
+        }
+        ss_adjoint_exp(g2, psi, lam, mask, d2, a.status);
+        ss_adjoint_exp(g1, psi, lam, mask, d1, a.status);
+#pragma unroll
+        for (int k = 0; k < SS_LIE; ++k) {
+            dfa[k] = 0.5 * dt * d1[k];
+            dfb[k] = 0.5 * dt * d2[k];
+        }
+#endif
+        ss_unframe(dfa, rot, tra, lab);
+        ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
+        ss_unframe(dfb, rot, trb, lab);
+        ss_scatter_sample(a, &ctx, trb + tc, lab, win, base, grad_row);
+#endif
+    }
+    for (int c = 0; c < a.n_controls; ++c) {
+        const long long size = ctx.tables[a.controls[c].table].size;
+#pragma unroll
+        for (int s = 0; s < SS_ADJ_WINDOW; ++s) {
+            const long long idx = base[c] + s;
+            const double v = win[c * SS_ADJ_WINDOW + s];
+            if (v != 0.0 && idx >= 0 && idx < size) atomicAdd(grad_row + a.controls[c].grad_offset + idx, v);
+        }
+    }
+}

@@ -49,6 +49,7 @@ int fail(const char* fmt, ...)
 
 constexpr int kAbiVersion = 2;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
+static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
 constexpr int kScanChunk = 8;   // matrices per lane in the state scan
 constexpr int kScanBlock = 256;
 
@@ -67,6 +68,7 @@ struct ss_module {
     hipModule_t module = nullptr;
     hipFunction_t single = nullptr;
     hipFunction_t batch = nullptr;
+    hipFunction_t adjoint = nullptr;
     int dim = 2;
     int method = 0;
     int expo = 0;
@@ -251,9 +253,27 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply(const double* u, long l
     }
 }
 
+// Step k of a simulation's scan: the forward state scan applies U[o+k] and
+// stores at o+k; the costate scan walks backward, applying U[o+n-1-k]^+ and
+// storing at o+n-1-k (costate[i] = U[i+1]^+ costate[i+1], costate[n-1] given).
+template <int D, bool REV>
+__device__ __forceinline__ void scan_step_mat(const double* u, long long o0, long long n, long long k, ss_cd m[D][D])
+{
+    if (!REV) {
+        load_mat<D>(u, o0 + k, m);
+    } else {
+        ss_cd t[D][D];
+        load_mat<D>(u, o0 + n - 1 - k, t);
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) m[r][c] = ss_conj(t[c][r]);
+    }
+}
+
 // One wave per simulation of a batch: the same three phases within a wave.
-template <int D>
-__global__ void __launch_bounds__(64) batch_scan(const double* psi0, const double* u, double* state,
+template <int D, bool REV>
+__global__ void __launch_bounds__(64) batch_scan(const double* init, const double* u, double* out,
                                                    const long long* offsets)
 {
     const long long sim = blockIdx.x;
@@ -269,24 +289,51 @@ __global__ void __launch_bounds__(64) batch_scan(const double* psi0, const doubl
     ss_set_identity<D>(x);
     for (long long k = k0; k < kend_m; ++k) {
         ss_cd m[D][D];
-        load_mat<D>(u, o0 + k, m);
+        scan_step_mat<D, REV>(u, o0, n, k, m);
         ss_left_multiply<D>(m, x);
     }
     ss_cd ex[D][D];
     wave_scan<D>(x, ex, lane);
     ss_cd p[D], v[D];
-    load_vec<D>(psi0, sim, p);
+    load_vec<D>(init, sim, p);
     ss_matvec<D>(ex, p, v);
     for (long long k = k0; k < kend_s; ++k) {
-        store_vec<D>(state, o0 + k, v);
+        store_vec<D>(out, REV ? o0 + n - 1 - k : o0 + k, v);
         if (k + 1 < kend_s) {
             ss_cd m[D][D], t[D];
-            load_mat<D>(u, o0 + k, m);
+            scan_step_mat<D, REV>(u, o0, n, k, m);
             ss_matvec<D>(m, v, t);
 #pragma unroll
             for (int r = 0; r < D; ++r) v[r] = t[r];
         }
     }
+}
+
+// psi(T) of each simulation: U[last] state[last] (psi0 when it has no rows).
+template <int D>
+__global__ void __launch_bounds__(256) final_state_kernel(const double* psi0, const double* u, const double* state,
+                                                            const long long* offsets, double* out, long long n_sims)
+{
+    const long long sim = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sim >= n_sims) return;
+    const long long o1 = offsets[sim + 1];
+    ss_cd v[D], r[D];
+    if (o1 == offsets[sim]) {
+        load_vec<D>(psi0, sim, r);
+    } else {
+        ss_cd m[D][D];
+        load_mat<D>(u, o1 - 1, m);
+        load_vec<D>(state, o1 - 1, v);
+        ss_matvec<D>(m, v, r);
+    }
+    store_vec<D>(out, sim, r);
+}
+
+__global__ void __launch_bounds__(256) zero_doubles(double* p, long long n, int* flag)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+    if (i == 0 && flag) flag[0] = 0;
 }
 
 // get_spin, spinsim/__init__.py:936-944.
@@ -483,6 +530,7 @@ int ss_module_load(const void* code, size_t code_size, int dim, int method, int 
     if (e == hipSuccess) e = hipModuleLoadData(&m->module, code);
     if (e == hipSuccess) e = hipModuleGetFunction(&m->single, m->module, "ss_time_evolution_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&m->batch, m->module, "ss_batch_time_evolution_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&m->adjoint, m->module, "ss_batch_adjoint_kernel");
     if (e != hipSuccess) {
         if (m->module) (void)hipModuleUnload(m->module);
         delete m;
@@ -686,10 +734,111 @@ int ss_batch_state_scan(const double* psi0, const double* u, double* state, cons
     hipStream_t s = static_cast<hipStream_t>(stream);
     const long long* off = reinterpret_cast<const long long*>(offsets);
     if (dim == 2)
-        hipLaunchKernelGGL(batch_scan<2>, dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
+        hipLaunchKernelGGL((batch_scan<2, false>), dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
     else
-        hipLaunchKernelGGL(batch_scan<3>, dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
+        hipLaunchKernelGGL((batch_scan<3, false>), dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
     SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_batch_costate_scan(const double* lambda_final, const double* u, double* costate, const int64_t* offsets,
+                          int64_t n_sims, int dim, ss_stream stream)
+{
+    if (dim != 2 && dim != 3) return fail("ss_batch_costate_scan: dim must be 2 or 3");
+    if (n_sims < 0) return fail("ss_batch_costate_scan: negative n_sims");
+    if (n_sims == 0) return 0;
+    if (!lambda_final || !u || !costate || !offsets) return fail("ss_batch_costate_scan: null buffer");
+    if (n_sims > 2147483647LL) return fail("ss_batch_costate_scan: too many simulations");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long* off = reinterpret_cast<const long long*>(offsets);
+    if (dim == 2)
+        hipLaunchKernelGGL((batch_scan<2, true>), dim3(unsigned(n_sims)), dim3(64), 0, s, lambda_final, u, costate,
+                           off);
+    else
+        hipLaunchKernelGGL((batch_scan<3, true>), dim3(unsigned(n_sims)), dim3(64), 0, s, lambda_final, u, costate,
+                           off);
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_batch_final_state(const double* psi0, const double* u, const double* state, const int64_t* offsets,
+                         double* out, int64_t n_sims, int dim, ss_stream stream)
+{
+    if (dim != 2 && dim != 3) return fail("ss_batch_final_state: dim must be 2 or 3");
+    if (n_sims < 0) return fail("ss_batch_final_state: negative n_sims");
+    if (n_sims == 0) return 0;
+    if (!psi0 || !u || !state || !offsets || !out) return fail("ss_batch_final_state: null buffer");
+    const long long blocks = (n_sims + 255) / 256;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long* off = reinterpret_cast<const long long*>(offsets);
+    if (dim == 2)
+        hipLaunchKernelGGL(final_state_kernel<2>, dim3(unsigned(blocks)), dim3(256), 0, s, psi0, u, state, off, out,
+                           (long long)n_sims);
+    else
+        hipLaunchKernelGGL(final_state_kernel<3>, dim3(unsigned(blocks)), dim3(256), 0, s, psi0, u, state, off, out,
+                           (long long)n_sims);
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stride, int64_t n_sims,
+                     const int64_t* offsets, const double* t0, const double* dt, const double* dt_out,
+                     const int32_t* n_fine, int64_t total, const double* u, const double* state,
+                     const double* costate, const ss_table* tables, const ss_control_desc* controls,
+                     int n_controls, double* grad, int64_t grad_stride, int32_t* status, ss_stream stream)
+{
+    if (!m) return fail("ss_batch_adjoint: null module");
+    if (n_sims < 0 || total < 0) return fail("ss_batch_adjoint: negative size");
+    if (n_controls < 1 || n_controls > SS_MAX_CONTROLS)
+        return fail("ss_batch_adjoint: 1 to %d controls, got %d", SS_MAX_CONTROLS, n_controls);
+    if (!controls || !tables || !grad || !status) return fail("ss_batch_adjoint: null buffer");
+    for (int c = 0; c < n_controls; ++c) {
+        const int ax = controls[c].axis;
+        if (ax < 0 || ax > 3 || (m->dim == 2 && ax == 3))
+            return fail("ss_batch_adjoint: control %d has axis %d (x=0, y=1, z=2, q=3 for spin one)", c, ax);
+        if (controls[c].table < 0) return fail("ss_batch_adjoint: control %d has a negative table", c);
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long n_grad = (long long)n_sims * grad_stride;
+    const long long zb = (std::max(n_grad, 1LL) + 255) / 256;
+    hipLaunchKernelGGL(zero_doubles, dim3(unsigned(zb)), dim3(256), 0, s, grad, n_grad, status);
+    SS_HIP(hipGetLastError());
+    if (n_sims == 0 || total == 0) return 0;
+    if (!sweep || !offsets || !t0 || !dt || !dt_out || !n_fine || !u || !state || !costate)
+        return fail("ss_batch_adjoint: null buffer");
+    ss_adjoint_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.sweep = sweep;
+    a.tables = tables;
+    a.offsets = reinterpret_cast<const long long*>(offsets);
+    a.t0 = t0;
+    a.dt = dt;
+    a.dt_out = dt_out;
+    a.n_fine = n_fine;
+    a.u = u;
+    a.state = state;
+    a.costate = costate;
+    a.grad = grad;
+    a.status = status;
+    a.total = total;
+    a.n_sims = n_sims;
+    a.sweep_stride = sweep_stride;
+    a.grad_stride = grad_stride;
+    double oa = 0, ob = 0;
+    sample_offsets(m->method, 2.0, &oa, &ob);
+    a.off_a_frac = oa;
+    a.off_b_frac = ob;
+    a.n_controls = n_controls;
+    for (int c = 0; c < n_controls; ++c) {
+        a.controls[c].axis = controls[c].axis;
+        a.controls[c].table = controls[c].table;
+        a.controls[c].scale = controls[c].scale;
+        a.controls[c].grad_offset = controls[c].grad_offset;
+    }
+    const long long blocks = (total + 255) / 256;
+    if (blocks > 2147483647LL) return fail("ss_batch_adjoint: grid too large");
+    void* params[] = {&a};
+    SS_HIP(hipModuleLaunchKernel(m->adjoint, unsigned(blocks), 1, 1, 256, 1, 1, 0, s, params, nullptr));
     return 0;
 }
 
