@@ -260,14 +260,16 @@ def _sample(get_field, t, sweep, lie, n):
 def get_time_evolution(get_field, spin, time_start, time_end, time_step_integration,
                        time_step_output, sweep_parameters=(0.0,), exponentiation=None,
                        integration=MAGNUS_CF4, use_rotating_frame=True, number_of_squares=24,
-                       coarse_index=None):
+                       coarse_index=None, record=None):
     """Return (time (N,), U (N, d, d) complex128, dt, n_fine).
 
     ``get_field(t, sweep, field)`` is evaluated with ``t`` a float64 array of
     lanes and ``field`` a (lie_dimension, lanes) array, so a field written with
     NumPy ufuncs follows the reference's per-sample callback (:620) lane by
     lane.  ``coarse_index`` restricts the run to a subset of coarse steps (the
-    result rows then follow that subset).
+    result rows then follow that subset).  ``record``, if a list, receives one
+    dict per fine step (CF4 only): the lab sample times, windings, and the
+    generators g of its two exponentials, lanes along the last axis.
     """
     dim, lie, exponentiation, integration = resolve_methods(spin, exponentiation, integration)
     dt, n_coarse, n_fine, warned = timing(time_start, time_end, time_step_integration,
@@ -311,8 +313,13 @@ def get_time_evolution(get_field, spin, time_start, time_end, time_step_integrat
             transform_frame(fb, rotating_wave, wb, dim, use_rotating_frame)
             g = [dt * (w0 * fa[k] + w1 * fb[k]) for k in range(lie)]
             u = append(g)
-            g = [dt * (w1 * fa[k] + w0 * fb[k]) for k in range(lie)]
-            u = append(g)
+            g2 = [dt * (w1 * fa[k] + w0 * fb[k]) for k in range(lie)]
+            u = append(g2)
+            if record is not None:
+                record.append({"ta": (time_fine + 0.5 * dt * (1 - 1 / SQRT3) - time_coarse)
+                               + time_coarse, "tb": t_rel + time_coarse, "wa": wa, "wb": wb,
+                               "g1": np.array(g), "g2": np.array(g2), "dt": dt,
+                               "rot": rotating_wave})
         elif integration == HEUN:                                  # :692-752
             t_rel = time_fine - time_coarse
             wa = np.exp(1j * rotating_wave * t_rel)

@@ -6,13 +6,14 @@ Drop-in for the reference package's public API (spinsim/__init__.py):
 ``generate_interpolation_sampler``.
 """
 from .enums import Device, ExponentiationMethod, IntegrationMethod, SpinQuantumNumber
+from .adjoint import Control, ControlledSimulator
 from .batch import BatchPlan, BatchResults
 from .interpolation import InterpolationSampler, generate_interpolation_sampler
 from .simulator import FieldSource, Results, Simulator
 from .transpile import TranspileError, transpile
 
 __all__ = [
-    "BatchPlan", "BatchResults", "Device", "ExponentiationMethod", "IntegrationMethod", "SpinQuantumNumber",
+    "BatchPlan", "BatchResults", "Control", "ControlledSimulator", "Device", "ExponentiationMethod", "IntegrationMethod", "SpinQuantumNumber",
     "InterpolationSampler", "generate_interpolation_sampler", "FieldSource", "Results",
     "Simulator", "TranspileError", "transpile",
 ]

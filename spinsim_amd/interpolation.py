@@ -49,6 +49,7 @@ class InterpolationSampler:
         self.zero_boundary = bool(zero_boundary)
         self.version = 0
         self._device = {}
+        self._bound = None
 
     @property
     def rows(self) -> int:
@@ -62,6 +63,17 @@ class InterpolationSampler:
         self.amplitude = amplitude
         self.version += 1
         self._device.clear()
+
+    def bind(self, amplitude_rows, stride: int):
+        """Read the amplitudes from a device tensor view: row s of the table starts
+        at ``amplitude_rows.data_ptr() + s*stride`` doubles (no copy).  Used for
+        differentiable control amplitudes."""
+        self._bound = (amplitude_rows, int(stride))
+        self.version += 1
+
+    def unbind(self):
+        self._bound = None
+        self.version += 1
 
     def device_arrays(self, device):
         """(time, amplitude) as float64 tensors on ``device`` (cached)."""
@@ -82,6 +94,10 @@ class InterpolationSampler:
         rec.amplitude = a.data_ptr()
         rec.size = self.time.size
         rec.amplitude_stride = self.time.size if self.amplitude.ndim == 2 else 0
+        bound = getattr(self, "_bound", None)
+        if bound is not None:
+            rec.amplitude = bound[0].data_ptr()
+            rec.amplitude_stride = bound[1]
         rec.order = self.order
         rec.zero_boundary = int(self.zero_boundary)
         return rec
