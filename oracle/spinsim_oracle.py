@@ -14,11 +14,13 @@ time, residual "m1" squarings, the left-multiplication order), so that the
 only differences from the reference are libm ulps and NumPy's complex
 division.
 
-How each function is pinned (the tests that would catch a dropped term, a sign,
-an index or a transposed operand) is listed in ``oracle/README.md``; every
-function here has at least one such check, except the bit-level float
-behaviour of numba/libdevice, which no test can pin and which sits far below
-the 1e-9 tolerance.
+Each function is pinned by checks independent of its own formulas, in
+``tests/test_oracle_pinning.py``: scipy ``expm`` (exponentials), an adaptive
+ODE solution and convergence orders (sampling, CF4 and frame), closed forms
+(Larmor, the Gaussian pi-pulse of gaussian_example.py), eigenstate spins,
+polynomial reproduction and scipy's CubicHermiteSpline (sampler), known step
+counts.  Parity unpinned: the bit-level float behaviour of numba/libdevice,
+which no test can reach and which sits far below the 1e-9 tolerance.
 """
 from __future__ import annotations
 
