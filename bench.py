@@ -8,13 +8,17 @@ cos(2pi*700 kHz t), z = 2pi*700 kHz, quadratic shift 2pi*1 kHz; t = 0 -> 100 ms,
 integration step 100 ns, output step 1 us: 1e5 coarse x 10 fine = 1e6 CF4 steps;
 Lie-Trotter exponentials (24 squares), rotating frame, plain float64.  A step
 is one evaluate() pass over inputs resident in HBM: the time-evolution kernel,
-the state prefix product and the Bloch vectors (outputs stay on the device).
+the state prefix product with the Bloch vectors fused into its last pass
+(outputs stay on the device).
 
 Also reported (``variants``): the same workload with float32-bulk step
-operators, and the batched ensemble of 4096 spin-half Rabi simulations with
+operators; the batched ensemble of 4096 spin-half Rabi simulations with
 uneven durations (1-50 ms, 50 ns steps, 1 us outputs; float32-bulk; one
-persistent launch + per-simulation state scans replayed as a HIP graph),
-each with its state error against the float64 oracle.  Prints one JSON line.
+persistent launch + per-simulation state scans replayed as a HIP graph), each
+with its state error against the float64 oracle; and the differentiation
+workload (1024 spin-one pulses, forward + adjoint gradient, float64 and
+float32-bulk forward), with its gradient error against central differences.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -97,8 +101,7 @@ def single_workload(ss, torch, precision, steps, warmup, lanes):
         sim.launch_time_evolution(sweep, T0, dt, DT_OUT, time_out, u, lanes)
         if events is not None:
             events[1].record(stream)
-        sim.launch_state_scan(psi0, u, state, ws)
-        sim.launch_spin(state, spin)
+        sim.launch_state_scan(psi0, u, state, ws, spin=spin)
 
     for _ in range(warmup):
         step()
@@ -132,7 +135,7 @@ def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
     dur = rng.uniform(1e-3, 50e-3, n_sims)
     sim = ss.Simulator(RABI_SWEEP, ss.SpinQuantumNumber.HALF, precision="fp32_bulk")
     psi0 = ss.SpinQuantumNumber.HALF.plus_z
-    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep)
+    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep, with_spin=True)
     tune_batch(plan, reps=1, warmup=0, lanes=(1, 2, 4), items=(1, 4, 16))
     plan.capture()
     for _ in range(warmup):
@@ -156,6 +159,67 @@ def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
             "lanes_per_step": plan.lanes, "items_per_grab": plan.items_per_grab,
             "state_err_vs_oracle_sampled": max(errs), "sampled_simulations": check,
             "dtype": "f32 residuals / f64 samples, products, scan", "graph": True}
+
+
+def _z_bias(time_sample, sweep_parameters, field_sample):
+    field_sample[2] = 2 * math.pi * 1e5
+
+
+def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_coords=8):
+    """1024 spin-one state-transfer pulses (2 ms, 20 ns steps, 2000 cubic control
+    samples on x and y), loss 1 - |psi(T)[-1]|^2: forward + adjoint gradient,
+    checked against central differences of the float64 forward."""
+    import numpy as np
+
+    rng = np.random.default_rng(20261202)
+    t_end, m, sigma = 2e-3, 2000, 2 * math.pi * 5e3
+    tk = np.linspace(0.0, t_end, m)
+    amp = rng.normal(0, sigma, (n_sims, 2, m))
+    psi0 = ss.SpinQuantumNumber.ONE.plus_z
+    cs = ss.ControlledSimulator(ss.SpinQuantumNumber.ONE, [ss.Control("x", tk),
+                                                           ss.Control("y", tk)], _z_bias,
+                                precision=precision)
+    amps = torch.tensor(amp, device="cuda")
+
+    def run():
+        a = amps.clone().requires_grad_(True)
+        psi = cs.final_state(a, 0.0, t_end, 20e-9, 1e-6, psi0)
+        (1 - psi[:, -1].abs() ** 2).sum().backward()
+        return a.grad
+
+    grad = run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        grad = run()
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t) / reps
+    steps = n_sims * 100000
+    # central differences (float64 forward), all perturbations in one batch
+    ref = cs if precision == "fp64" else ss.ControlledSimulator(
+        ss.SpinQuantumNumber.ONE, [ss.Control("x", tk), ss.Control("y", tk)], _z_bias)
+    h = 1e-4 * sigma
+    coords, rows = [], []
+    for s_ in rng.choice(n_sims, fd_sims, False):
+        for _ in range(fd_coords):
+            k, j = int(rng.integers(2)), int(rng.integers(m))
+            coords.append((int(s_), k, j))
+            for sign in (1, -1):
+                a = amp[s_].copy()
+                a[k, j] += sign * h
+                rows.append(a)
+    with torch.no_grad():
+        psi_p = ref.final_state(torch.tensor(np.stack(rows), device="cuda"), 0.0, t_end, 20e-9,
+                                1e-6, psi0).cpu().numpy()
+    loss_p = 1 - np.abs(psi_p[:, -1]) ** 2
+    fd = (loss_p[0::2] - loss_p[1::2]) / (2 * h)
+    g = grad.cpu().numpy()
+    ad = np.array([g[s_, k, j] for (s_, k, j) in coords])
+    rel = float(np.abs(fd - ad).max() / np.abs(fd).max())
+    return {"forward_backward_ms": elapsed * 1e3, "steps_per_s": 2 * steps / elapsed,
+            "fine_steps_per_pass": steps, "simulations": n_sims, "control_samples": 2 * m,
+            "grad_rel_err_vs_central_fd": rel, "fd_coordinates": len(coords),
+            "precision": precision}
 
 
 def main():
@@ -193,6 +257,8 @@ def main():
                                                       "lanes_per_step")}
         variants["batched_4096_spin_half"] = batched_workload(ss, torch, max(3, args.steps // 4),
                                                               1)
+        for prec in ("fp64", "fp32_bulk"):
+            variants[f"adjoint_1024_spin_one_{prec}"] = adjoint_workload(ss, torch, prec)
 
     if args.copy_probe:
         src = torch.empty(args.copy_probe << 17, dtype=torch.float64, device="cuda")

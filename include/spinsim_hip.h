@@ -139,11 +139,13 @@ int ss_batch_time_evolution(const ss_module *m, const double *sweep, int64_t swe
 size_t ss_state_scan_workspace(int64_t n, int dim);
 /* state[0] = psi0, state[i] = u[i-1] state[i-1] for i < n (u[n-1] unused),
  * by a parallel prefix product in float64.  psi0: HOST array of dim complex
- * values; u, state, workspace: device. */
-int ss_state_scan(const double *psi0, const double *u, double *state, int64_t n, int dim,
-                  void *workspace, size_t workspace_bytes, ss_stream stream);
-/* Per-simulation state scan over CSR rows; psi0: device (n_sims, dim). */
-int ss_batch_state_scan(const double *psi0, const double *u, double *state,
+ * values; u, state, workspace: device.  spin: NULL, or a device (n, 3) array
+ * that also receives the Bloch vectors (as ss_spin) in the same pass. */
+int ss_state_scan(const double *psi0, const double *u, double *state, double *spin, int64_t n,
+                  int dim, void *workspace, size_t workspace_bytes, ss_stream stream);
+/* Per-simulation state scan over CSR rows; psi0: device (n_sims, dim);
+ * spin: NULL or device (total, 3). */
+int ss_batch_state_scan(const double *psi0, const double *u, double *state, double *spin,
                         const int64_t *offsets, int64_t n_sims, int dim, ss_stream stream);
 
 /* Costates for the adjoint: costate[o+n-1] = lambda_final[s] and

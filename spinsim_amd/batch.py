@@ -73,7 +73,8 @@ class BatchResults:
     @property
     def spin(self):
         if self._spin is None:
-            self._spin = self._plan.sim._spin_calculator(self.state)
+            out = getattr(self._plan, "spin_out", None)
+            self._spin = out if out is not None else self._plan.sim._spin_calculator(self.state)
         return self._spin
 
     def __len__(self):
@@ -89,13 +90,14 @@ class BatchResults:
 class BatchPlan:
     """Device buffers and launch parameters for one batched workload.
 
-    ``run()`` enqueues time evolution + state scan on the current stream;
+    ``run()`` enqueues time evolution + state scan (+ Bloch vectors into
+    ``spin_out`` when ``with_spin``) on the current stream;
     ``capture()`` records them into a HIP graph and ``replay()`` re-runs it.
     """
 
     def __init__(self, sim, time_start, time_end, time_step_integration, time_step_output,
                  state_init, sweep_parameters, *, lanes_per_step=None, items_per_grab=None,
-                 grid_blocks=0):
+                 grid_blocks=0, with_spin=False):
         torch = sim._torch()
         self.sim = sim
         d = sim.dimension
@@ -142,6 +144,7 @@ class BatchPlan:
         self.items_per_grab = int(items_per_grab or 4)
         self.grid_blocks = int(grid_blocks)
         self.graph = None
+        self.spin_out = (torch.empty((total, 3), dtype=f64, device=dev) if with_spin else None)
 
     def run_time_evolution(self, stream=None):
         torch = self.sim._torch()
@@ -160,10 +163,11 @@ class BatchPlan:
         torch = self.sim._torch()
         if stream is None:
             stream = torch.cuda.current_stream(self.device).cuda_stream
+        spin = getattr(self, "spin_out", None)
         _native.check(_native.lib().ss_batch_state_scan(
             self.psi0.data_ptr(), self.u.data_ptr(), self.state.data_ptr(),
-            self.offsets.data_ptr(), self.n_sims, self.sim.dimension, stream),
-            "ss_batch_state_scan")
+            None if spin is None else spin.data_ptr(), self.offsets.data_ptr(), self.n_sims,
+            self.sim.dimension, stream), "ss_batch_state_scan")
 
     def run(self):
         self.run_time_evolution()

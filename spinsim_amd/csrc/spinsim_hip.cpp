@@ -47,10 +47,10 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 2;
+constexpr int kAbiVersion = 3;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
-constexpr int kScanChunk = 8;   // matrices per lane in the state scan
+constexpr int kScanChunk = 1;   // matrices per lane in the state scan
 constexpr int kScanBlock = 256;
 
 uint64_t fnv1a(const char* s, uint64_t h)
@@ -121,6 +121,40 @@ __device__ __forceinline__ void store_vec(double* s, long long k, const ss_cd v[
     double2* p = reinterpret_cast<double2*>(s) + k * D;
 #pragma unroll
     for (int r = 0; r < D; ++r) p[r] = make_double2(v[r].x, v[r].y);
+}
+
+// get_spin, spinsim/__init__.py:936-944, for one state.
+template <int D>
+__device__ __forceinline__ void store_spin(const ss_cd s[D], double* spin, long long i)
+{
+    double sx, sy, sz;
+    if (D == 2) {
+        const ss_cd p = ss_mul(s[0], ss_conj(s[1]));
+        sx = p.x;
+        sy = -p.y;  // Re(i psi0 psi1*)
+        sz = 0.5 * (s[0].x * s[0].x + s[0].y * s[0].y - s[1].x * s[1].x - s[1].y * s[1].y);
+    } else {
+        const double r2 = 1.4142135623730951;
+        const ss_cd c1 = ss_conj(s[1]);
+        const ss_cd a = ss_mul(c1, ss_add(s[0], s[D - 1]));
+        const ss_cd b = ss_mul(c1, ss_c(s[0].x - s[D - 1].x, s[0].y - s[D - 1].y));
+        sx = 2.0 * a.x / r2;
+        sy = -2.0 * b.y / r2;  // Re(2i b)/sqrt2
+        sz = s[0].x * s[0].x + s[0].y * s[0].y - s[D - 1].x * s[D - 1].x - s[D - 1].y * s[D - 1].y;
+    }
+    spin[3 * i + 0] = sx;
+    spin[3 * i + 1] = sy;
+    spin[3 * i + 2] = sz;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) spin_kernel(const double* state, double* spin, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ss_cd s[D];
+    load_vec<D>(state, i, s);
+    store_spin<D>(s, spin, i);
 }
 
 template <int D>
@@ -231,7 +265,7 @@ __global__ void __launch_bounds__(kScanBlock) scan_blocks(const double* block_to
 // walks its chunk, writing the states.
 template <int D>
 __global__ void __launch_bounds__(kScanBlock) scan_apply(const double* u, long long n, const double* lane_prefix,
-                                                           const double* carry, double* state)
+                                                           const double* carry, double* state, double* spin)
 {
     const long long lane_id = (long long)blockIdx.x * kScanBlock + threadIdx.x;
     const long long k0 = lane_id * kScanChunk;
@@ -243,6 +277,7 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply(const double* u, long l
     ss_matvec<D>(pre, c, v);
     for (long long k = k0; k < kend; ++k) {
         store_vec<D>(state, k, v);
+        if (spin) store_spin<D>(v, spin, k);
         if (k + 1 < kend) {
             ss_cd m[D][D], t[D];
             load_mat<D>(u, k, m);
@@ -271,17 +306,18 @@ __device__ __forceinline__ void scan_step_mat(const double* u, long long o0, lon
     }
 }
 
-// One wave per simulation of a batch: the same three phases within a wave.
+// One workgroup per simulation of a batch: the same three phases within it
+// (lane chunks, workgroup scan, walk).
 template <int D, bool REV>
-__global__ void __launch_bounds__(64) batch_scan(const double* init, const double* u, double* out,
-                                                   const long long* offsets)
+__global__ void __launch_bounds__(kScanBlock) batch_scan(const double* init, const double* u, double* out,
+                                                           const long long* offsets, double* spin)
 {
     const long long sim = blockIdx.x;
     const int lane = threadIdx.x;
     const long long o0 = offsets[sim];
     const long long n = offsets[sim + 1] - o0;
     if (n <= 0) return;
-    const long long chunk = (n + 63) / 64;
+    const long long chunk = (n + kScanBlock - 1) / kScanBlock;
     const long long k0 = lane * chunk;
     const long long kend_m = k0 + chunk < n - 1 ? k0 + chunk : n - 1;
     const long long kend_s = k0 + chunk < n ? k0 + chunk : n;
@@ -292,13 +328,14 @@ __global__ void __launch_bounds__(64) batch_scan(const double* init, const doubl
         scan_step_mat<D, REV>(u, o0, n, k, m);
         ss_left_multiply<D>(m, x);
     }
-    ss_cd ex[D][D];
-    wave_scan<D>(x, ex, lane);
+    ss_cd ex[D][D], total[D][D];
+    block_scan<D>(x, ex, total);
     ss_cd p[D], v[D];
     load_vec<D>(init, sim, p);
     ss_matvec<D>(ex, p, v);
     for (long long k = k0; k < kend_s; ++k) {
         store_vec<D>(out, REV ? o0 + n - 1 - k : o0 + k, v);
+        if (spin) store_spin<D>(v, spin, REV ? o0 + n - 1 - k : o0 + k);
         if (k + 1 < kend_s) {
             ss_cd m[D][D], t[D];
             scan_step_mat<D, REV>(u, o0, n, k, m);
@@ -336,33 +373,6 @@ __global__ void __launch_bounds__(256) zero_doubles(double* p, long long n, int*
     if (i == 0 && flag) flag[0] = 0;
 }
 
-// get_spin, spinsim/__init__.py:936-944.
-template <int D>
-__global__ void __launch_bounds__(256) spin_kernel(const double* state, double* spin, long long n)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    ss_cd s[D];
-    load_vec<D>(state, i, s);
-    double sx, sy, sz;
-    if (D == 2) {
-        const ss_cd p = ss_mul(s[0], ss_conj(s[1]));
-        sx = p.x;
-        sy = -p.y;  // Re(i psi0 psi1*)
-        sz = 0.5 * (s[0].x * s[0].x + s[0].y * s[0].y - s[1].x * s[1].x - s[1].y * s[1].y);
-    } else {
-        const double r2 = 1.4142135623730951;
-        const ss_cd c1 = ss_conj(s[1]);
-        const ss_cd a = ss_mul(c1, ss_add(s[0], s[D - 1]));
-        const ss_cd b = ss_mul(c1, ss_c(s[0].x - s[D - 1].x, s[0].y - s[D - 1].y));
-        sx = 2.0 * a.x / r2;
-        sy = -2.0 * b.y / r2;  // Re(2i b)/sqrt2
-        sz = s[0].x * s[0].x + s[0].y * s[0].y - s[D - 1].x * s[D - 1].x - s[D - 1].y * s[D - 1].y;
-    }
-    spin[3 * i + 0] = sx;
-    spin[3 * i + 1] = sy;
-    spin[3 * i + 2] = sz;
-}
 
 // Evaluate sampler k of `tables` at n times (row `sim` of a batched table).
 __global__ void __launch_bounds__(256) interp_kernel(const dev::ss_table* tables, int k, int sim, const double* t,
@@ -691,8 +701,8 @@ size_t ss_state_scan_workspace(int64_t n, int dim)
     return scan_layout(n, dim).bytes;
 }
 
-int ss_state_scan(const double* psi0, const double* u, double* state, int64_t n, int dim, void* workspace,
-                  size_t workspace_bytes, ss_stream stream)
+int ss_state_scan(const double* psi0, const double* u, double* state, double* spin, int64_t n, int dim,
+                  void* workspace, size_t workspace_bytes, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_state_scan: dim must be 2 or 3");
     if (n < 0) return fail("ss_state_scan: negative n");
@@ -713,18 +723,18 @@ int ss_state_scan(const double* psi0, const double* u, double* state, int64_t n,
     if (dim == 2) {
         hipLaunchKernelGGL(scan_lanes<2>, grid, block, 0, s, u, (long long)n, prefix, total);
         hipLaunchKernelGGL(scan_blocks<2>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
-        hipLaunchKernelGGL(scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state);
+        hipLaunchKernelGGL(scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
     } else {
         hipLaunchKernelGGL(scan_lanes<3>, grid, block, 0, s, u, (long long)n, prefix, total);
         hipLaunchKernelGGL(scan_blocks<3>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
-        hipLaunchKernelGGL(scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state);
+        hipLaunchKernelGGL(scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
     }
     SS_HIP(hipGetLastError());
     return 0;
 }
 
-int ss_batch_state_scan(const double* psi0, const double* u, double* state, const int64_t* offsets,
-                        int64_t n_sims, int dim, ss_stream stream)
+int ss_batch_state_scan(const double* psi0, const double* u, double* state, double* spin,
+                        const int64_t* offsets, int64_t n_sims, int dim, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_state_scan: dim must be 2 or 3");
     if (n_sims < 0) return fail("ss_batch_state_scan: negative n_sims");
@@ -734,9 +744,11 @@ int ss_batch_state_scan(const double* psi0, const double* u, double* state, cons
     hipStream_t s = static_cast<hipStream_t>(stream);
     const long long* off = reinterpret_cast<const long long*>(offsets);
     if (dim == 2)
-        hipLaunchKernelGGL((batch_scan<2, false>), dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
+        hipLaunchKernelGGL((batch_scan<2, false>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, psi0, u, state,
+                           off, spin);
     else
-        hipLaunchKernelGGL((batch_scan<3, false>), dim3(unsigned(n_sims)), dim3(64), 0, s, psi0, u, state, off);
+        hipLaunchKernelGGL((batch_scan<3, false>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, psi0, u, state,
+                           off, spin);
     SS_HIP(hipGetLastError());
     return 0;
 }
@@ -752,11 +764,11 @@ int ss_batch_costate_scan(const double* lambda_final, const double* u, double* c
     hipStream_t s = static_cast<hipStream_t>(stream);
     const long long* off = reinterpret_cast<const long long*>(offsets);
     if (dim == 2)
-        hipLaunchKernelGGL((batch_scan<2, true>), dim3(unsigned(n_sims)), dim3(64), 0, s, lambda_final, u, costate,
-                           off);
+        hipLaunchKernelGGL((batch_scan<2, true>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, lambda_final, u,
+                           costate, off, nullptr);
     else
-        hipLaunchKernelGGL((batch_scan<3, true>), dim3(unsigned(n_sims)), dim3(64), 0, s, lambda_final, u, costate,
-                           off);
+        hipLaunchKernelGGL((batch_scan<3, true>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, lambda_final, u,
+                           costate, off, nullptr);
     SS_HIP(hipGetLastError());
     return 0;
 }

@@ -293,16 +293,18 @@ class Simulator:
             time_out.data_ptr(), u_out.data_ptr(), self._table_ptr(dev), int(lanes), stream),
             "ss_time_evolution")
 
-    def launch_state_scan(self, psi0, u, state, workspace, stream=None):
-        """Enqueue ss_state_scan; psi0 is a host complex128 array of d values."""
+    def launch_state_scan(self, psi0, u, state, workspace, spin=None, stream=None):
+        """Enqueue ss_state_scan (and the Bloch vectors into ``spin`` when given);
+        psi0 is a host complex128 array of d values."""
         torch = self._torch()
         n, d = int(u.shape[0]), self.dimension
         psi = np.ascontiguousarray(np.asarray(psi0, np.complex128).reshape(d))
         if stream is None:
             stream = torch.cuda.current_stream(u.device).cuda_stream
-        _native.check(_native.lib().ss_state_scan(psi.ctypes.data, u.data_ptr(), state.data_ptr(),
-                                                  n, d, workspace.data_ptr(), workspace.numel(),
-                                                  stream), "ss_state_scan")
+        _native.check(_native.lib().ss_state_scan(
+            psi.ctypes.data, u.data_ptr(), state.data_ptr(),
+            None if spin is None else spin.data_ptr(), n, d, workspace.data_ptr(),
+            workspace.numel(), stream), "ss_state_scan")
 
     def launch_spin(self, state, spin, stream=None):
         torch = self._torch()

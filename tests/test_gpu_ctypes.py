@@ -27,7 +27,7 @@ def _libs():
     ss.ss_time_evolution.argtypes = [vp, vp, dbl, dbl, dbl, i64, vp, vp, vp, c_int, vp]
     ss.ss_state_scan_workspace.restype = ctypes.c_size_t
     ss.ss_state_scan_workspace.argtypes = [i64, c_int]
-    ss.ss_state_scan.argtypes = [vp, vp, vp, i64, c_int, vp, ctypes.c_size_t, vp]
+    ss.ss_state_scan.argtypes = [vp, vp, vp, vp, i64, c_int, vp, ctypes.c_size_t, vp]
     ss.ss_spin.argtypes = [vp, vp, i64, c_int, vp]
     hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
     hip.hipFree.argtypes = [vp]
@@ -72,7 +72,8 @@ def test_raw_ctypes_binding_matches_oracle():
     ws = ss.ss_state_scan_workspace(n, 3)
     d_ws = _Dev(hip, ws)
     psi0 = np.ascontiguousarray(O.EIGENSTATES["one"]["plus_x"])
-    assert ss.ss_state_scan(psi0.ctypes.data, d_u.ptr, d_s.ptr, n, 3, d_ws.ptr, ws, None) == 0
+    assert ss.ss_state_scan(psi0.ctypes.data, d_u.ptr, d_s.ptr, None, n, 3, d_ws.ptr, ws,
+                            None) == 0
     assert ss.ss_spin(d_s.ptr, d_o.ptr, n, 3, None) == 0
     d_t.get(time), d_u.get(u), d_s.get(state), d_o.get(spin)
     for b in bufs + [d_ws]:

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == 2
+    assert lib.ss_abi_version() == 3
     assert lib.ss_source_hash() != 0
 
 
@@ -44,7 +44,7 @@ def test_library_rejects_bad_arguments():
     lib = _native.lib()
     assert lib.ss_time_evolution(None, None, 0.0, 1e-7, 1e-6, 10, None, None, None, 0, None) != 0
     assert b"null module" in lib.ss_last_error()
-    assert lib.ss_state_scan(None, None, None, 5, 4, None, 0, None) != 0
+    assert lib.ss_state_scan(None, None, None, None, 5, 4, None, 0, None) != 0
     assert b"dim" in lib.ss_last_error()
     assert lib.ss_spin(None, None, -1, 2, None) != 0
     assert lib.ss_state_scan_workspace(0, 2) == 0
