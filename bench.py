@@ -161,6 +161,72 @@ def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
             "dtype": "f32 residuals / f64 samples, products, scan", "graph": True}
 
 
+def long_noisy_inputs(seed=20261202):
+    """Spin-half long noisy run: z = 2pi (700 kHz + 8 Gaussian pulses) + 1/f noise,
+    x = 2pi 1 kHz cos(2pi 700 kHz t); t = 0 - 100 ms.  Returns (field, sweep, sampler
+    args): pulses in sweep (amplitudes Hz, centres s, widths s), the noise as a
+    cubic-Hermite table on a 1 us grid (std 2pi 50 rad/s, PSD ~ 1/f, no DC)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(1e3, 1e4, 8)
+    centre = rng.uniform(0, 0.1, 8)
+    width = rng.uniform(5e-6, 50e-6, 8)
+    grid = np.arange(100001) * 1e-6
+    spec = np.fft.rfft(rng.normal(size=grid.size))
+    freq = np.fft.rfftfreq(grid.size, 1e-6)
+    spec[0] = 0
+    spec[1:] /= np.sqrt(freq[1:])
+    noise = np.fft.irfft(spec, grid.size)
+    noise *= 2 * math.pi * 50 / noise.std()
+    return np.concatenate([amp, centre, width]), grid, noise
+
+
+def make_long_noisy_field(noise_sampler):
+    def get_field(time_sample, sweep_parameters, field_sample):
+        z = 7e5
+        for k in range(8):
+            d = time_sample - sweep_parameters[8 + k]
+            z += sweep_parameters[k] * math.exp(-d * d / (2 * sweep_parameters[16 + k] ** 2))
+        field_sample[2] = 2 * math.pi * z + noise_sampler(time_sample)
+        field_sample[0] = 2 * math.pi * 1e3 * math.cos(2 * math.pi * 7e5 * time_sample)
+        field_sample[1] = 0
+    return get_field
+
+
+def long_noisy_workload(ss, torch, reps=3):
+    """1e7 CF4 steps (dt 10 ns, output 1 us), float64 and float32-bulk: steps/s,
+    unitarity drift max |U^+ U - I| of the coarse operators, and the float32-bulk
+    state error against float64."""
+    import numpy as np
+
+    sweep, grid, noise = long_noisy_inputs()
+    sampler = ss.generate_interpolation_sampler(grid, noise)
+    field = make_long_noisy_field(sampler)
+    psi0 = ss.SpinQuantumNumber.HALF.plus_z
+    out, states = {}, {}
+    for prec in ("fp64", "fp32_bulk"):
+        sim = ss.Simulator(field, ss.SpinQuantumNumber.HALF, precision=prec)
+        res = sim.evaluate(0.0, 0.1, 10e-9, 1e-6, psi0, sweep, output="torch",
+                           lanes_per_step="auto")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            res = sim.evaluate(0.0, 0.1, 10e-9, 1e-6, psi0, sweep, output="torch",
+                               lanes_per_step="auto")
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t) / reps
+        u = res.time_evolution
+        eye = torch.eye(2, dtype=u.dtype, device=u.device)
+        drift = float((torch.matmul(u.conj().transpose(1, 2).contiguous(), u) - eye).abs().max())
+        states[prec] = res.state.cpu().numpy()
+        out[prec] = {"steps_per_s": 1e7 / el, "ms_per_evaluate": 1e3 * el,
+                     "unitarity_drift": drift}
+    out["fp32_bulk"]["state_err_vs_fp64"] = float(np.abs(states["fp32_bulk"] -
+                                                         states["fp64"]).max())
+    return out
+
+
 def _z_bias(time_sample, sweep_parameters, field_sample):
     field_sample[2] = 2 * math.pi * 1e5
 
@@ -259,6 +325,7 @@ def main():
                                                               1)
         for prec in ("fp64", "fp32_bulk"):
             variants[f"adjoint_1024_spin_one_{prec}"] = adjoint_workload(ss, torch, prec)
+        variants["long_noisy_1e7_spin_half"] = long_noisy_workload(ss, torch)
 
     if args.copy_probe:
         src = torch.empty(args.copy_probe << 17, dtype=torch.float64, device="cuda")

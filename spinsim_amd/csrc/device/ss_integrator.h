@@ -456,6 +456,29 @@ static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], 
 }
 #endif
 
+// Winding exp(i rot t_rel) of a sample, advanced from step to step by
+// multiplication: w(t') = w(t) exp(i rot dt) (1 + i rot ((t' - t) - dt)), where
+// t' - t is the exact difference of the rounded sample offsets the reference
+// forms, so w follows exp(i rot t_rel) of those offsets to ~1e-15 over a
+// chunk; it is re-anchored with sincos at the start of every lane's chunk.
+struct ss_wind {
+    ss_cd w;
+    double t_rel;
+};
+static __device__ __forceinline__ void ss_wind_advance(ss_wind& s, double t_rel_next, ss_cd step, double rot,
+                                                       double dt)
+{
+#if SS_ROT
+    const double eps = rot * ((t_rel_next - s.t_rel) - dt);
+    s.w = ss_mul(ss_mul(s.w, step), ss_c(1.0, eps));
+#else
+    (void)step;
+    (void)rot;
+    (void)dt;
+#endif
+    s.t_rel = t_rel_next;
+}
+
 // `count` fine steps starting at fine index j0 of the coarse step starting at
 // tc; u (initially identity) accumulates them.  The fine time is advanced by
 // repeated addition from tc exactly like the reference (:819, :836).
@@ -480,28 +503,31 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #else
 #define SS_APPEND(g) ss_append(g, u)
 #endif
+    if (count <= 0) return;
+    ss_wind wa, wb;
+    wa.t_rel = (tf + off_a) - tc;
+    wa.w = ss_winding(rot, wa.t_rel);
+    wb.t_rel = (tf + off_b) - tc;
+#if SS_METHOD != 1
+    wb.w = ss_winding(rot, wb.t_rel);
+#endif
+    const ss_cd step = ss_winding(rot, dt);
 #pragma unroll 1
     for (int j = 0; j < count; ++j) {
         double fa[SS_FIELD_LEN], g[SS_FIELD_LEN];
 #if SS_METHOD == 1
         // euler (midpoint sample), :758-791
-        const double tr = (tf + off_a) - tc;
-        const ss_cd wa = ss_winding(rot, tr);
-        ss_call_field(tr + tc, sweep, fa, ctx);
-        ss_frame(fa, rot, wa);
+        ss_call_field(wa.t_rel + tc, sweep, fa, ctx);
+        ss_frame(fa, rot, wa.w);
 #pragma unroll
         for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fa[k];
         SS_APPEND(g);
 #else
         double fb[SS_FIELD_LEN];
-        double tr = (tf + off_a) - tc;
-        const ss_cd wa = ss_winding(rot, tr);
-        ss_call_field(tr + tc, sweep, fa, ctx);
-        tr = (tf + off_b) - tc;
-        const ss_cd wb = ss_winding(rot, tr);
-        ss_call_field(tr + tc, sweep, fb, ctx);
-        ss_frame(fa, rot, wa);
-        ss_frame(fb, rot, wb);
+        ss_call_field(wa.t_rel + tc, sweep, fa, ctx);
+        ss_call_field(wb.t_rel + tc, sweep, fb, ctx);
+        ss_frame(fa, rot, wa.w);
+        ss_frame(fb, rot, wb.w);
 #if SS_METHOD == 0
         // magnus_cf4, :655-686
 #pragma unroll
@@ -521,9 +547,15 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #endif
 #endif
         tf += dt;
+        if (j + 1 < count) {
+            ss_wind_advance(wa, (tf + off_a) - tc, step, rot, dt);
+#if SS_METHOD != 1
+            ss_wind_advance(wb, (tf + off_b) - tc, step, rot, dt);
+#endif
+        }
     }
 #if SS_PREC == 1
-    if (count > 0) ss_mixed_to_u(gsum, bres, u);
+    ss_mixed_to_u(gsum, bres, u);
 #endif
 #undef SS_APPEND
 }
