@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--lanes", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32_bulk"),
+                    help="arithmetic of the headline run (profiling the other variant)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
@@ -312,7 +314,8 @@ def main():
     from oracle.c_oracle import COracle
 
     torch.cuda.set_device(0)
-    main_run, state, psi0 = single_workload(ss, torch, "fp64", args.steps, args.warmup, lanes)
+    main_run, state, psi0 = single_workload(ss, torch, args.precision, args.steps, args.warmup,
+                                            lanes)
     n, nf, dt = main_run["coarse"], main_run["n_fine"], main_run["dt"]
     achieved = main_run["fine"] * FLOPS_PER_STEP / (main_run["kernel_ms"] * 1e-3) / 1e12
 
@@ -352,7 +355,8 @@ def main():
     line = {
         "metric": METRIC, "value": main_run["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_run["ms_per_step"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 residuals / f64",
         "data": "synthetic",
         "config": {"workload": "spin-one Rabi drive, Lie-Trotter (24 squares), rotating frame, "
                                "plain float64, t 0-100 ms, dt 100 ns, output 1 us",
