@@ -528,23 +528,25 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
         ss_call_field(wb.t_rel + tc, sweep, fb, ctx);
         ss_frame(fa, rot, wa.w);
         ss_frame(fb, rot, wb.w);
+        // both exponents first, so the samples are dead before the exponentials
+        double g2[SS_FIELD_LEN];
 #if SS_METHOD == 0
         // magnus_cf4, :655-686
 #pragma unroll
-        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * (SS_W0 * fa[k] + SS_W1 * fb[k]);
-        SS_APPEND(g);
-#pragma unroll
-        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * (SS_W1 * fa[k] + SS_W0 * fb[k]);
-        SS_APPEND(g);
+        for (int k = 0; k < SS_FIELD_LEN; ++k) {
+            g[k] = dt * (SS_W0 * fa[k] + SS_W1 * fb[k]);
+            g2[k] = dt * (SS_W1 * fa[k] + SS_W0 * fb[k]);
+        }
 #else
         // heun (half step), :724-752
 #pragma unroll
-        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fa[k] / 2;
-        SS_APPEND(g);
-#pragma unroll
-        for (int k = 0; k < SS_FIELD_LEN; ++k) g[k] = dt * fb[k] / 2;
-        SS_APPEND(g);
+        for (int k = 0; k < SS_FIELD_LEN; ++k) {
+            g[k] = dt * fa[k] / 2;
+            g2[k] = dt * fb[k] / 2;
+        }
 #endif
+        SS_APPEND(g);
+        SS_APPEND(g2);
 #endif
         tf += dt;
         if (j + 1 < count) {
