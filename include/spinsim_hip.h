@@ -42,8 +42,10 @@ typedef struct ss_module ss_module;
 typedef void *ss_stream; /* hipStream_t */
 
 /* One interpolation table of generate_interpolation_sampler
- * (spinsim/__init__.py:1941-2142), in device memory.  amplitude_stride > 0
- * selects row `sim` of a (B, size) amplitude array in batched launches. */
+ * (spinsim/__init__.py:1941-2142), in device memory: `size` >= 2 ascending
+ * times (a table of fewer samples reads as zero, or amplitude[0] when not
+ * zero_boundary).  amplitude_stride > 0 selects row `sim` of a (B, size)
+ * amplitude array in batched launches. */
 typedef struct ss_table {
     const double *time;
     const double *amplitude;

@@ -9,11 +9,11 @@ from .enums import Device, ExponentiationMethod, IntegrationMethod, SpinQuantumN
 from .adjoint import Control, ControlledSimulator
 from .batch import BatchPlan, BatchResults
 from .interpolation import InterpolationSampler, generate_interpolation_sampler
-from .simulator import FieldSource, Results, Simulator
+from .simulator import FieldSource, Results, Simulator, precision_error
 from .transpile import TranspileError, transpile
 
 __all__ = [
     "BatchPlan", "BatchResults", "Control", "ControlledSimulator", "Device", "ExponentiationMethod", "IntegrationMethod", "SpinQuantumNumber",
     "InterpolationSampler", "generate_interpolation_sampler", "FieldSource", "Results",
-    "Simulator", "TranspileError", "transpile",
+    "Simulator", "TranspileError", "precision_error", "transpile",
 ]

@@ -871,6 +871,10 @@ static __device__ __forceinline__ void ss_interp_weights(const ss_table* tb, dou
     const long long size = tb->size;
     w[0] = w[1] = w[2] = w[3] = 0.0;
     *first = 0;
+    if (size < 2) {  // degenerate table: a constant (or zero) field
+        if (size == 1 && !tb->zero_boundary) w[0] = 1.0;
+        return;
+    }
     if (!tb->zero_boundary) {
         if (t <= time[0]) { w[0] = 1.0; return; }
         if (t >= time[size - 1]) { *first = size - 1; w[0] = 1.0; return; }

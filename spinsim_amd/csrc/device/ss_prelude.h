@@ -88,6 +88,7 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
     const double* time = tb->time;
     const double* amp = tb->amplitude + (long long)ctx->sim * tb->amplitude_stride;
     const long long size = tb->size;
+    if (size < 2) return (size == 1 && !tb->zero_boundary) ? amp[0] : 0.0;  // degenerate table
     if (!tb->zero_boundary) {
         if (t <= time[0]) return amp[0];
         if (t >= time[size - 1]) return amp[size - 1];

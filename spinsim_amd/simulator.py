@@ -410,3 +410,31 @@ class Simulator:
         from .batch import evaluate_batch
         return evaluate_batch(self, time_start, time_end, time_step_integration,
                               time_step_output, state_init, sweep_parameters, **kw)
+
+
+def precision_error(sim: Simulator, time_start, time_end, time_step_integration, time_step_output,
+                    state_init, sweep_parameters=[0], **kw) -> dict:
+    """Error of a float32-bulk Simulator against float64 on the same inputs.
+
+    Builds a float64 twin of ``sim`` (same field, spin, methods and frame), runs
+    both on the device and returns the maximum absolute differences of the state
+    and of the coarse step operators, and the float32-bulk run's unitarity
+    deviation max |U^+ U - I|.
+    """
+    import torch
+
+    twin = Simulator(sim.field, sim.spin_quantum_number,
+                     exponentiation_method=sim.exponentiation_method,
+                     use_rotating_frame=sim.use_rotating_frame,
+                     integration_method=sim.integration_method,
+                     number_of_squares=sim.number_of_squares, precision="fp64")
+    args = (time_start, time_end, time_step_integration, time_step_output, state_init,
+            sweep_parameters)
+    a = sim.evaluate(*args, output="torch", **kw)
+    b = twin.evaluate(*args, output="torch", **kw)
+    u = a.time_evolution
+    eye = torch.eye(sim.dimension, dtype=u.dtype, device=u.device)
+    unitarity = (torch.matmul(u.conj().transpose(1, 2).contiguous(), u) - eye).abs()
+    return {"state": float((a.state - b.state).abs().max()) if a.state.numel() else 0.0,
+            "time_evolution": float((u - b.time_evolution).abs().max()) if u.numel() else 0.0,
+            "unitarity": float(unitarity.max()) if u.numel() else 0.0}

@@ -86,3 +86,13 @@ def test_fp32_bulk_full_rabi_one_within_1e6():
     dev = np.abs(np.einsum("nij,nkj->nik", uu, uu.conj()) - np.eye(3)).max()
     print(f"  max |U U^+ - I| = {dev:.3e}")
     assert dev < 1e-9
+
+
+def test_precision_error_helper():
+    import spinsim_amd as ss
+
+    print("case: precision_error", flush=True)
+    sim = ss.Simulator(F.TILTED_ONE.src, ss.SpinQuantumNumber.ONE, precision="fp32_bulk")
+    err = ss.precision_error(sim, 0.0, 4e-4, 1e-7, 1e-6, O.EIGENSTATES["one"]["plus_x"])
+    print(f"  {err}")
+    assert 0 < err["state"] <= 1e-8 and err["time_evolution"] <= 1e-8 and err["unitarity"] <= 1e-9
