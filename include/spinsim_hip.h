@@ -119,8 +119,11 @@ int ss_module_info(const ss_module *m, int *vgprs, int *sgprs, int *scratch_byte
  * floor(dt_out/dt + 0.5) step exponentials of [time_out[i], time_out[i] +
  * dt_out), taken out of the rotating frame.  `dt` is the integration step as
  * evaluate() re-derives it.  sweep: device array read by get_field; tables:
- * device ss_table array or NULL.  lanes_per_step in 1..64 lanes share one
- * coarse step (0 = library default).  u_out: n_coarse*dim*dim complex128. */
+ * device ss_table array or NULL.  lanes_per_step in 1..64: that many lanes
+ * share one coarse step; 0 (the default): as many waves as fit on the device
+ * at once, with rows dealt in phases of decreasing item length (lanes per
+ * step) so all waves finish together; -k (1..64): the same on k waves per
+ * SIMD.  u_out: n_coarse*dim*dim complex128. */
 int ss_time_evolution(const ss_module *m, const double *sweep, double t0, double dt,
                       double dt_out, int64_t n_coarse, double *time_out, double *u_out,
                       const ss_table *tables, int lanes_per_step, ss_stream stream);

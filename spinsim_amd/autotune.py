@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the time-evolution kernels.
 
-Chooses how many lanes share one coarse step (and, for batches, how many
-rows a wave takes per grab of the work counter) by timing the candidates on
-the device with HIP events.  Results are cached per (kernel, shape, device)
-in memory and in a JSON file next to the code-object cache.
+Chooses between the phased default launch and a number of lanes sharing one
+coarse step (and, for batches, how many rows a wave takes per grab of the
+work counter) by timing the candidates on the device with HIP events.
+Results are cached per (kernel, shape, device) in memory and in a JSON file
+next to the code-object cache.
 """
 from __future__ import annotations
 
@@ -82,10 +83,13 @@ def time_launch(fn, reps=3, warmup=1):
 
 def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep, t0: float = 0.0,
                reps: int = 3, use_cache: bool = True):
-    """Best lanes_per_step for one simulation of this shape; returns (lanes, table)."""
+    """Best lanes_per_step for one simulation of this shape (0 / -1: phased
+    launch on the resident waves / one wave per SIMD, else lanes per coarse
+    step); returns (lanes, table)."""
     import torch
 
-    key = _key(sim, "single", n_coarse, n_fine)
+    candidates = [0, -1] + lane_candidates(n_fine)
+    key = _key(sim, "single", n_coarse, n_fine, tuple(candidates))
     if use_cache:
         with _lock:
             if key in _memo:
@@ -102,7 +106,7 @@ def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep,
     sw = sweep if isinstance(sweep, torch.Tensor) else torch.as_tensor(sweep, dtype=torch.float64,
                                                                         device=dev)
     table = {}
-    for g in lane_candidates(n_fine):
+    for g in candidates:
         table[g] = time_launch(
             lambda: sim.launch_time_evolution(sw, t0, dt, dt_out, time_out, u, g), reps=reps)
     best = min(table, key=table.get)

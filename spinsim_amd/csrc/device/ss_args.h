@@ -2,10 +2,14 @@
 // integrator.  Plain C layout, 8-byte aligned, passed by value.
 #pragma once
 
-// Single-simulation time-evolution launch: one coarse (output) step per group
-// of `lanes_per_step` lanes, mirroring the reference kernel
+#define SS_MAX_PHASES 8
+
+// Single-simulation time-evolution launch, mirroring the reference kernel
 // get_time_evolution(sweep, time_coarse, time_end_points, dt, dt_out, U)
-// (spinsim/__init__.py:854-896).
+// (spinsim/__init__.py:854-896).  The coarse rows are cut into phases; in
+// phase p a group of phase_lanes[p] lanes shares one coarse step, so an item
+// (one wave's worth of groups) covers 64 / phase_lanes[p] rows.  Items are
+// numbered phase after phase and dealt to the `waves` waves round-robin.
 struct ss_te_args {
     const double* sweep;         // sweep_parameters (device)
     const void* tables;          // ss_table[] (device) or null
@@ -18,7 +22,12 @@ struct ss_te_args {
     double off_a;                // first sample offset from the fine-step start
     double off_b;                // second sample offset (CF4 / HEUN)
     int n_fine;                  // floor(dt_out/dt + 0.5)
-    int lanes_per_step;          // lanes sharing one coarse step (1..64)
+    int n_phases;                // 1..SS_MAX_PHASES
+    long long waves;             // waves launched
+    long long items;             // items over all phases
+    int phase_lanes[SS_MAX_PHASES];
+    long long phase_row[SS_MAX_PHASES];    // first row of each phase
+    long long phase_items[SS_MAX_PHASES];  // items of each phase
 };
 
 // Batched launch: simulation s owns coarse rows [offsets[s], offsets[s+1]).

@@ -627,33 +627,45 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
     time_out[row] = tc;
 }
 
-// One simulation: get_time_evolution, spinsim/__init__.py:854-896.
+// One simulation: get_time_evolution, spinsim/__init__.py:854-896.  Wave w
+// takes items w, w + waves, w + 2 waves, ... of the phase table; within an
+// item, group gi of `lanes` lanes integrates coarse row
+// phase_row + (item - first item of the phase) * (64 / lanes) + gi.  Phases
+// with fewer lanes per row (long items) come first and the finest last, so
+// every wave ends within about one fine step of the others.
 extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te_args a)
 {
-    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const long long wave = gtid >> 6;
-    const int lanes = a.lanes_per_step;
-    const int groups = 64 / lanes;
-    const int gi = lane / lanes;
-    const int rank = lane - gi * lanes;
-    const long long row = wave * groups + gi;
-    const bool active = gi < groups && row < a.n_coarse;
-    const int base = a.n_fine / lanes;
-    const int rem = a.n_fine - base * lanes;
-    const int j0 = rank * base + (rank < rem ? rank : rem);
-    const int count = active ? base + (rank < rem ? 1 : 0) : 0;
-
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     ss_ctx ctx;
     ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
     ctx.sim = 0;
-    const double tc = ss_coarse_time(a.t0, a.dt_out, row);
-    const double rot = active ? ss_rotating_wave(tc, a.dt_out, a.sweep, &ctx) : 0.0;
-
-    ss_cd u[SS_D][SS_D];
-    ss_integrate(u, j0, count, tc, rot, a.dt, a.off_a, a.off_b, a.sweep, &ctx);
-    ss_group_reduce(u, rank, lanes);
-    if (active && rank == 0) ss_store(u, rot, a.dt_out, tc, row, a.time_out, a.u_out);
+#pragma unroll 1
+    for (long long item = wave; item < a.items; item += a.waves) {
+        int p = 0;
+        long long first = 0;
+        while (p + 1 < a.n_phases && item >= first + a.phase_items[p]) {
+            first += a.phase_items[p];
+            ++p;
+        }
+        const int lanes = a.phase_lanes[p];
+        const int groups = 64 / lanes;
+        const int gi = lane / lanes;
+        const int rank = lane - gi * lanes;
+        const long long end = p + 1 < a.n_phases ? a.phase_row[p + 1] : a.n_coarse;
+        const long long row = a.phase_row[p] + (item - first) * groups + gi;
+        const bool active = gi < groups && row < end;
+        const int base = a.n_fine / lanes;
+        const int rem = a.n_fine - base * lanes;
+        const int j0 = rank * base + (rank < rem ? rank : rem);
+        const int count = active ? base + (rank < rem ? 1 : 0) : 0;
+        const double tc = ss_coarse_time(a.t0, a.dt_out, row);
+        const double rot = active ? ss_rotating_wave(tc, a.dt_out, a.sweep, &ctx) : 0.0;
+        ss_cd u[SS_D][SS_D];
+        ss_integrate(u, j0, count, tc, rot, a.dt, a.off_a, a.off_b, a.sweep, &ctx);
+        ss_group_reduce(u, rank, lanes);
+        if (active && rank == 0) ss_store(u, rot, a.dt_out, tc, row, a.time_out, a.u_out);
+    }
 }
 
 // Many simulations in one persistent launch.  Each wave repeatedly takes

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <vector>
 
 #include "spinsim_hip.h"
 #include "device/ss_args.h"
@@ -74,6 +75,8 @@ struct ss_module {
     int expo = 0;
     int rotating = 1;
     int device = 0;
+    long long resident_waves = 0;  // single-simulation kernel: waves resident at once
+    long long simds = 0;           // SIMDs of the device (4 per CU)
 };
 
 // ---------------------------------------------------------------------------
@@ -411,24 +414,6 @@ ScanLayout scan_layout(int64_t n, int dim)
     return l;
 }
 
-int default_lanes(int64_t n_coarse, int n_fine)
-{
-    // The smallest divisor of n_fine (equal chunks per lane) that gives about
-    // eight waves per SIMD of an MI355X (256 CUs x 4 SIMDs); else the largest
-    // divisor up to 64, or a power of two when n_fine has no useful divisor.
-    const long long target = 256LL * 4 * 8 * 64;
-    int best = 1;
-    for (int g = 1; g <= 64 && g <= n_fine; ++g) {
-        if (n_fine % g) continue;
-        best = g;
-        if (n_coarse * g >= target) return g;
-    }
-    if (n_coarse * best * 4 >= target) return best;
-    int p = 1;
-    while (p < 64 && 2 * p <= n_fine && n_coarse * p < target) p *= 2;
-    return std::max(p, best);
-}
-
 std::string build_source(const char* field_src, int dim, int method, int expo, int rotating,
                          int number_of_squares, int field_has_context, int precision)
 {
@@ -541,11 +526,16 @@ int ss_module_load(const void* code, size_t code_size, int dim, int method, int 
     if (e == hipSuccess) e = hipModuleGetFunction(&m->single, m->module, "ss_time_evolution_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&m->batch, m->module, "ss_batch_time_evolution_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&m->adjoint, m->module, "ss_batch_adjoint_kernel");
+    int per_cu = 0, cus = 0;
+    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->single, 256, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device);
     if (e != hipSuccess) {
         if (m->module) (void)hipModuleUnload(m->module);
         delete m;
         return fail("ss_module_load: %s", hipGetErrorString(e));
     }
+    m->resident_waves = std::max(1LL, (long long)per_cu * cus * 4);
+    m->simds = std::max(1LL, (long long)cus * 4);
     *out = m;
     return 0;
 }
@@ -607,6 +597,53 @@ static void sample_offsets(int method, double dt, double* off_a, double* off_b)
     }
 }
 
+// Phase table for a single simulation on `waves` waves.  Candidate lanes per
+// coarse step G are the divisors of n_fine up to 64 (and the powers of two
+// when n_fine > 64, where no G gives one step per lane); an
+// item of G covers 64/G rows and costs ceil(n_fine/G) fine steps per lane.
+// Whole rounds (one item per wave) of the coarser G are taken while they fit
+// the ideal per-lane budget N n_fine / (64 waves); the remaining rows go to
+// the finest G, whose items cost the fewest steps, so waves finish within
+// about one such item of each other.
+static void plan_phases(long long n_coarse, int n_fine, long long waves, ss_te_args* a)
+{
+    std::vector<int> cand;
+    const int top = std::max(1, std::min(n_fine, 64));
+    for (int g = 1; g <= top; ++g)
+        if (n_fine % g == 0 || (n_fine > 64 && (g & (g - 1)) == 0)) cand.push_back(g);
+    const double budget = double(n_coarse) * n_fine / (64.0 * double(waves));
+    double used = 0;
+    long long row = 0;
+    int n = 0;
+    for (size_t c = 0; c + 1 < cand.size() && n + 1 < SS_MAX_PHASES; ++c) {
+        const int g = cand[c];
+        const long long rows = 64 / g;
+        const long long cost = (n_fine + g - 1) / g;
+        if (cost <= 0) continue;
+        long long k = (long long)std::floor((budget - used) / double(cost));
+        k = std::min(k, (n_coarse - row) / (rows * waves));
+        if (k <= 0) continue;
+        a->phase_lanes[n] = g;
+        a->phase_row[n] = row;
+        a->phase_items[n] = k * waves;
+        row += k * rows * waves;
+        used += double(k * cost);
+        ++n;
+    }
+    if (row < n_coarse || n == 0) {
+        const int g = cand.back();
+        const long long rows = 64 / g;
+        a->phase_lanes[n] = g;
+        a->phase_row[n] = row;
+        a->phase_items[n] = (n_coarse - row + rows - 1) / rows;
+        ++n;
+    }
+    a->n_phases = n;
+    a->items = 0;
+    for (int p = 0; p < n; ++p) a->items += a->phase_items[p];
+    a->waves = std::max(1LL, std::min(waves, a->items));
+}
+
 int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double dt, double dt_out,
                       int64_t n_coarse, double* time_out, double* u_out, const ss_table* tables,
                       int lanes_per_step, ss_stream stream)
@@ -619,9 +656,11 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
     const double nf = std::floor(dt_out / dt + 0.5);  // :832
     if (!(nf >= 0) || nf > 2147483647.0) return fail("ss_time_evolution: bad fine-step count");
     const int n_fine = int(nf);
-    if (lanes_per_step <= 0) lanes_per_step = default_lanes(n_coarse, n_fine);
-    if (lanes_per_step > 64) return fail("ss_time_evolution: lanes_per_step must be <= 64");
+    if (lanes_per_step > 64 || lanes_per_step < -64)
+        return fail("ss_time_evolution: lanes_per_step must be in -64..64");
+    if (n_coarse > (1LL << 40)) return fail("ss_time_evolution: n_coarse too large");
     ss_te_args a;
+    std::memset(&a, 0, sizeof(a));
     a.sweep = sweep;
     a.tables = tables;
     a.time_out = time_out;
@@ -632,10 +671,18 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
     a.dt_out = dt_out;
     sample_offsets(m->method, dt, &a.off_a, &a.off_b);
     a.n_fine = n_fine;
-    a.lanes_per_step = lanes_per_step;
-    const long long groups = 64 / lanes_per_step;
-    const long long waves = (n_coarse + groups - 1) / groups;
-    const long long blocks = (waves + 3) / 4;
+    if (lanes_per_step > 0) {
+        const long long groups = 64 / lanes_per_step;
+        a.n_phases = 1;
+        a.phase_lanes[0] = lanes_per_step;
+        a.phase_items[0] = (n_coarse + groups - 1) / groups;
+        a.items = a.phase_items[0];
+        a.waves = a.items;
+    } else {
+        const long long cap = lanes_per_step < 0 ? (long long)(-lanes_per_step) * m->simds : m->resident_waves;
+        plan_phases(n_coarse, n_fine, cap, &a);
+    }
+    const long long blocks = (a.waves + 3) / 4;
     if (blocks > 2147483647LL) return fail("ss_time_evolution: grid too large");
     void* params[] = {&a};
     SS_HIP(hipModuleLaunchKernel(m->single, unsigned(blocks), 1, 1, 256, 1, 1, 0,

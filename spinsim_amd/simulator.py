@@ -184,9 +184,11 @@ class Simulator:
     """Drop-in for spinsim.Simulator (spinsim/__init__.py:334-1111).
 
     Extra keyword arguments: ``samplers`` (for HIP-C field text),
-    ``lanes_per_step`` (lanes sharing one coarse step: an int in 1..64,
-    ``"auto"`` to time the candidates on first use of each shape, or None for
-    the library's divisor-of-n_fine default) and
+    ``lanes_per_step`` (lanes sharing one coarse step: an int in 1..64;
+    None or 0 for the default launch, which deals the rows to the resident
+    waves in phases of decreasing item length so they finish together; -k
+    for the same on k waves per SIMD; ``"auto"`` to time the candidates on
+    first use of each shape) and
     ``precision``: ``"fp64"`` (default; the reference's arithmetic) or
     ``"fp32_bulk"`` -- the higher-order residual exp(A) - I - A of each step
     operator and the products of residuals within a lane's chunk of steps in

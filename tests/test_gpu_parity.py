@@ -64,7 +64,7 @@ def assert_close(res, ref, tol=TOL):
 
 
 def test_larmor_full_size_and_closed_form():
-    """spin-half Larmor, 1e6 steps; also the exact answer."""
+    """Spin-half Larmor precession, 1e6 steps; also the exact answer."""
     print("case: larmor full size", flush=True)
     psi0 = O.EIGENSTATES["half"]["plus_x"]
     res = run_product(F.LARMOR, 0, 100e-3, 100e-9, 500e-9, psi0)
@@ -82,7 +82,7 @@ TAU1000 = 2 * math.pi * 1000
 
 
 def test_rabi_one_full_size():
-    """spin-one Rabi drive, Lie-Trotter, 1e6 steps, plain FP64."""
+    """Spin-one Rabi drive, Lie-Trotter, 1e6 steps, plain FP64."""
     print("case: rabi_one full size", flush=True)
     psi0 = O.EIGENSTATES["one"]["plus_z"]
     res = run_product(F.RABI_ONE, 0, 100e-3, 100e-9, 1e-6, psi0)
@@ -123,6 +123,32 @@ def test_lanes_per_step_partition(lanes):
     res = run_product(*args, lanes=lanes)
     ref = run_oracle(*args)
     assert_close(res, ref)
+
+
+PHASED = [
+    # (t1, dt, dt_out): n_fine 200 over 3 rows (powers of two, uneven split);
+    # one row; one fine step per row; uneven rows per phase (n_fine 7);
+    # n_fine 20 with several phases
+    (3e-4, 5e-7, 1e-4),
+    (1e-6, 1e-7, 1e-6),
+    (1e-3, 1e-6, 1e-6),
+    (7.0003e-3, 1e-7, 7e-7),
+    (2e-4, 5e-8, 1e-6),
+]
+
+
+@pytest.mark.parametrize("t1,dt,dt_out", PHASED)
+def test_phased_launch(t1, dt, dt_out):
+    """The default launch (rows dealt to the resident waves in phases of
+    decreasing lanes-per-step item length) matches the oracle, also on one
+    wave per SIMD."""
+    print(f"case: phased t1={t1} dt={dt} dt_out={dt_out}", flush=True)
+    psi0 = O.EIGENSTATES["one"]["plus_y"]
+    args = (F.TILTED_ONE, 0, t1, dt, dt_out, psi0)
+    ref = run_oracle(*args)
+    assert_close(run_product(*args, lanes=0), ref)
+    assert_close(run_product(*args, lanes=-1), ref)
+    assert_close(run_product(*args), ref)
 
 
 @pytest.mark.parametrize("squares", [0, 1, 7, 24])
