@@ -44,8 +44,10 @@ SIGNATURES = {
     "ss_state_scan_workspace": (ctypes.c_size_t, [c_i64, c_int]),
     "ss_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, ctypes.c_size_t,
                               c_vp]),
-    "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp]),
-    "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp]),
+    "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp,
+                                    ctypes.c_size_t, c_vp]),
+    "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp,
+                                      ctypes.c_size_t, c_vp]),
     "ss_batch_final_state": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp]),
     "ss_batch_adjoint": (c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_vp]),

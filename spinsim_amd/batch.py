@@ -145,6 +145,8 @@ class BatchPlan:
         self.grid_blocks = int(grid_blocks)
         self.graph = None
         self.spin_out = (torch.empty((total, 3), dtype=f64, device=dev) if with_spin else None)
+        ws = _native.lib().ss_state_scan_workspace(total, d) if total else 0
+        self.scan_ws = torch.empty(max(int(ws), 1), dtype=torch.uint8, device=dev)
 
     def run_time_evolution(self, stream=None):
         torch = self.sim._torch()
@@ -167,7 +169,8 @@ class BatchPlan:
         _native.check(_native.lib().ss_batch_state_scan(
             self.psi0.data_ptr(), self.u.data_ptr(), self.state.data_ptr(),
             None if spin is None else spin.data_ptr(), self.offsets.data_ptr(), self.n_sims,
-            self.sim.dimension, stream), "ss_batch_state_scan")
+            self.total, self.sim.dimension, self.scan_ws.data_ptr(), self.scan_ws.numel(), stream),
+            "ss_batch_state_scan")
 
     def run(self):
         self.run_time_evolution()

@@ -16,7 +16,6 @@
 #include <cstring>
 #include <string>
 #include <vector>
-#include <vector>
 
 #include "spinsim_hip.h"
 #include "device/ss_args.h"
@@ -48,11 +47,9 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 3;
+constexpr int kAbiVersion = 4;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
-constexpr int kScanChunk = 1;   // matrices per lane in the state scan
-constexpr int kScanBlock = 256;
 
 uint64_t fnv1a(const char* s, uint64_t h)
 {
@@ -160,192 +157,471 @@ __global__ void __launch_bounds__(256) spin_kernel(const double* state, double* 
     store_spin<D>(s, spin, i);
 }
 
+// ---- state scan: get_state (:1087-1111) as a two-pass chunked scan --------
+//
+// Rows are taken in scan order (forward for states, backward for costates).
+// Row r maps the previous row's state to its own: v -> U v with U the matrix
+// leading into r, or, where r starts a simulation, v -> init (an affine map
+// v -> m v + c with m = 0).  The rows are cut into chunks of whole 256-row
+// tiles, one workgroup per chunk.  Pass 1 composes each chunk's maps (a wave
+// shuffle reduction per tile, tiles in order) and stores the chunk's map.
+// Pass 2 finds each chunk's incoming state from the maps of the chunks before
+// it, back to the nearest one in which a simulation starts (up to 256 chunks
+// per look, multiplied in order by a shuffle tree), then walks its own tiles:
+// a wave shuffle scan plus the four wave totals through LDS gives each row its
+// state from the tile's incoming one, written with its Bloch vector.  No
+// workgroup waits on another: pass 2 starts after pass 1 has finished.
+
+constexpr int kScanThreads = 256;
+
+struct ScanArgs {
+    const double* u;            // (total, D, D)
+    double* out;                // (total, D) states, or costates
+    double* spin;               // (total, 3) or null
+    const long long* offsets;   // (n_sims + 1,) CSR rows, or null: one simulation
+    const double* init;         // (n_sims, D) device, or null: psi0 below
+    double psi0[6];
+    long long total;
+    long long n_sims;
+    long long chunk_rows;       // a multiple of kScanThreads
+    long long n_chunks;
+    double* agg;                // (n_chunks, kAggStride(D)): m, c, constant flag
+};
+
+// Doubles per chunk record (m, c, flag), padded to whole 16-byte pairs.
+__host__ __device__ constexpr int scan_agg_stride(int d) { return 2 * d * d + 2 * d + 2; }
+
 template <int D>
-__device__ __forceinline__ void shfl_up_mat(const ss_cd m[D][D], ss_cd out[D][D], int s)
+__device__ __forceinline__ void load_mat_at(const double* p, ss_cd m[D][D])
 {
+    const double2* q = reinterpret_cast<const double2*>(p);
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            out[r][c].x = __shfl_up(m[r][c].x, s, 64);
-            out[r][c].y = __shfl_up(m[r][c].y, s, 64);
+            const double2 v = q[r * D + c];
+            m[r][c] = ss_c(v.x, v.y);
         }
 }
 
-// Inclusive scan over the 64 lanes of a wave of x_l (later lanes on the
-// left): q_l = x_l x_{l-1} ... x_0.  Then ex = q_{l-1} (identity on lane 0).
-template <int D>
-__device__ __forceinline__ void wave_scan(ss_cd q[D][D], ss_cd ex[D][D], int lane)
+// Start row, in scan order, of the i-th simulation in scan order, and that
+// simulation's index.
+template <bool REV>
+__device__ __forceinline__ long long scan_start(const ScanArgs& a, long long i)
 {
-#pragma unroll 1
-    for (int s = 1; s < 64; s <<= 1) {
-        ss_cd p[D][D];
-        shfl_up_mat<D>(q, p, s);
-        if (lane >= s) ss_right_multiply<D>(q, p);
-    }
-    shfl_up_mat<D>(q, ex, 1);
-    if (lane == 0) ss_set_identity<D>(ex);
+    return REV ? a.total - a.offsets[a.n_sims - i] : a.offsets[i];
 }
 
-// Workgroup-wide exclusive scan (256 lanes = 4 waves).  On return `ex` is
-// the product of the x of all earlier lanes of the workgroup and `total`
-// (valid on every lane) the product over the whole workgroup.
-template <int D>
-__device__ __forceinline__ void block_scan(const ss_cd x[D][D], ss_cd ex[D][D], ss_cd total[D][D])
+template <bool REV>
+__device__ __forceinline__ long long scan_sim(const ScanArgs& a, long long i)
 {
-    __shared__ ss_cd wave_tot[kScanBlock / 64][D][D];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    ss_cd q[D][D];
-    ss_copy<D>(x, q);
-    ss_cd wex[D][D];
-    wave_scan<D>(q, wex, lane);
-    if (lane == 63) ss_copy<D>(q, wave_tot[wave]);
-    __syncthreads();
-    ss_cd pre[D][D];
-    ss_set_identity<D>(pre);
-    for (int w = 0; w < wave; ++w) ss_left_multiply<D>(wave_tot[w], pre);
-    ss_matmul<D>(wex, pre, ex);
-    ss_set_identity<D>(total);
-    for (int w = 0; w < kScanBlock / 64; ++w) ss_left_multiply<D>(wave_tot[w], total);
+    return REV ? a.n_sims - 1 - i : i;
 }
 
-// Pass 1: lane L covers states [L*C, L*C + C) and the matrices it applies,
-// [L*C, min(L*C + C, n - 1)).  Store each lane's exclusive in-block prefix and
-// each block's total.
-template <int D>
-__global__ void __launch_bounds__(kScanBlock) scan_lanes(const double* u, long long n, double* lane_prefix,
-                                                           double* block_total)
+// First simulation (scan order) whose start row is at or after row r.
+template <bool REV>
+__device__ long long scan_first_sim(const ScanArgs& a, long long r)
 {
-    const long long lane_id = (long long)blockIdx.x * kScanBlock + threadIdx.x;
-    const long long k0 = lane_id * kScanChunk;
-    const long long kend = k0 + kScanChunk < n - 1 ? k0 + kScanChunk : n - 1;
-    ss_cd x[D][D];
-    ss_set_identity<D>(x);
-    for (long long k = k0; k < kend; ++k) {
-        ss_cd m[D][D];
-        load_mat<D>(u, k, m);
-        ss_left_multiply<D>(m, x);
+    long long lo = 0, hi = a.n_sims;
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (scan_start<REV>(a, mid) >= r) hi = mid; else lo = mid + 1;
     }
-    ss_cd ex[D][D], total[D][D];
-    block_scan<D>(x, ex, total);
-    store_mat<D>(lane_prefix, lane_id, ex);
-    if (threadIdx.x == 0) store_mat<D>(block_total, blockIdx.x, total);
+    return lo;
 }
 
-// Pass 2 (one workgroup): carry-in state of every block,
-// v_b = (B_{b-1} ... B_0) psi0.
 template <int D>
-__global__ void __launch_bounds__(kScanBlock) scan_blocks(const double* block_total, long long nb,
-                                                            ss_cd p0, ss_cd p1, ss_cd p2, double* carry)
+__device__ __forceinline__ void load_vec_c(const double* p, ss_cd v[D])
 {
-    const long long per = (nb + kScanBlock - 1) / kScanBlock;
-    const long long b0 = per * threadIdx.x;
-    const long long b1 = b0 + per < nb ? b0 + per : nb;
-    ss_cd x[D][D];
-    ss_set_identity<D>(x);
-    for (long long b = b0; b < b1; ++b) {
-        ss_cd m[D][D];
-        load_mat<D>(block_total, b, m);
-        ss_left_multiply<D>(m, x);
-    }
-    ss_cd ex[D][D], total[D][D];
-    block_scan<D>(x, ex, total);
-    ss_cd psi[3] = {p0, p1, p2};
-    ss_cd v[D];
-    ss_matvec<D>(ex, psi, v);
-    for (long long b = b0; b < b1; ++b) {
-        store_vec<D>(carry, b, v);
-        ss_cd m[D][D], t[D];
-        load_mat<D>(block_total, b, m);
-        ss_matvec<D>(m, v, t);
 #pragma unroll
-        for (int r = 0; r < D; ++r) v[r] = t[r];
-    }
+    for (int r = 0; r < D; ++r) v[r] = ss_c(p[2 * r], p[2 * r + 1]);
 }
 
-// Pass 3: each lane starts from (its in-block prefix) * (block carry) and
-// walks its chunk, writing the states.
-template <int D>
-__global__ void __launch_bounds__(kScanBlock) scan_apply(const double* u, long long n, const double* lane_prefix,
-                                                           const double* carry, double* state, double* spin)
-{
-    const long long lane_id = (long long)blockIdx.x * kScanBlock + threadIdx.x;
-    const long long k0 = lane_id * kScanChunk;
-    if (k0 >= n) return;
-    const long long kend = k0 + kScanChunk < n ? k0 + kScanChunk : n;
-    ss_cd pre[D][D], c[D], v[D];
-    load_mat<D>(lane_prefix, lane_id, pre);
-    load_vec<D>(carry, blockIdx.x, c);
-    ss_matvec<D>(pre, c, v);
-    for (long long k = k0; k < kend; ++k) {
-        store_vec<D>(state, k, v);
-        if (spin) store_spin<D>(v, spin, k);
-        if (k + 1 < kend) {
-            ss_cd m[D][D], t[D];
-            load_mat<D>(u, k, m);
-            ss_matvec<D>(m, v, t);
-#pragma unroll
-            for (int r = 0; r < D; ++r) v[r] = t[r];
-        }
-    }
-}
-
-// Step k of a simulation's scan: the forward state scan applies U[o+k] and
-// stores at o+k; the costate scan walks backward, applying U[o+n-1-k]^+ and
-// storing at o+n-1-k (costate[i] = U[i+1]^+ costate[i+1], costate[n-1] given).
+// Matrix leading into scan-order row r (r >= 1): U[r - 1] forward,
+// U[total - r]^+ backward (costate[i] = U[i+1]^+ costate[i+1]).
 template <int D, bool REV>
-__device__ __forceinline__ void scan_step_mat(const double* u, long long o0, long long n, long long k, ss_cd m[D][D])
+__device__ __forceinline__ void scan_row_mat(const ScanArgs& a, long long r, ss_cd m[D][D])
 {
     if (!REV) {
-        load_mat<D>(u, o0 + k, m);
+        load_mat<D>(a.u, r - 1, m);
     } else {
         ss_cd t[D][D];
-        load_mat<D>(u, o0 + n - 1 - k, t);
+        load_mat<D>(a.u, a.total - r, t);
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int k = 0; k < D; ++k) m[i][k] = ss_conj(t[k][i]);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void scan_init_vec(const ScanArgs& a, int s, ss_cd v[D])
+{
+    if (a.init) {
+        load_vec_c<D>(a.init + (long long)s * 2 * D, v);
+    } else {
+#pragma unroll
+        for (int r = 0; r < D; ++r) v[r] = ss_c(a.psi0[2 * r], a.psi0[2 * r + 1]);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void set_zero_vec(ss_cd v[D])
+{
+#pragma unroll
+    for (int r = 0; r < D; ++r) v[r] = ss_c(0.0, 0.0);
+}
+
+// Affine map v -> m v + c (c = 0 throughout when AFF is false).
+// (m, c) <- (lm, lc) o (m, c), in place a column at a time.
+template <int D, bool AFF>
+__device__ __forceinline__ void aff_after(const ss_cd lm[D][D], const ss_cd lc[D], ss_cd m[D][D], ss_cd c[D])
+{
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        ss_cd col[D];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            ss_cd acc = ss_mul(lm[r][0], m[0][k]);
+#pragma unroll
+            for (int i = 1; i < D; ++i) acc = ss_mac(acc, lm[r][i], m[i][k]);
+            col[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r) m[r][k] = col[r];
+    }
+    if (AFF) {
+        ss_cd v[D];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            ss_cd acc = lc[r];
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc = ss_mac(acc, lm[r][i], c[i]);
+            v[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r) c[r] = v[r];
+    }
+}
+
+// (m, c) <- (m, c) o (pm, pc), in place a row at a time.
+template <int D, bool AFF>
+__device__ __forceinline__ void aff_before(ss_cd m[D][D], ss_cd c[D], const ss_cd pm[D][D], const ss_cd pc[D])
+{
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        ss_cd row[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            ss_cd acc = ss_mul(m[r][0], pm[0][k]);
+#pragma unroll
+            for (int i = 1; i < D; ++i) acc = ss_mac(acc, m[r][i], pm[i][k]);
+            row[k] = acc;
+        }
+        if (AFF) {
+            ss_cd acc = c[r];
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc = ss_mac(acc, m[r][i], pc[i]);
+            c[r] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) m[r][k] = row[k];
+    }
+}
+
+template <int D, bool AFF, bool UP>
+__device__ __forceinline__ void shfl_aff(const ss_cd m[D][D], const ss_cd c[D], ss_cd om[D][D], ss_cd oc[D], int s)
+{
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            om[r][k].x = UP ? __shfl_up(m[r][k].x, s, 64) : __shfl_down(m[r][k].x, s, 64);
+            om[r][k].y = UP ? __shfl_up(m[r][k].y, s, 64) : __shfl_down(m[r][k].y, s, 64);
+        }
+        if (AFF) {
+            oc[r].x = UP ? __shfl_up(c[r].x, s, 64) : __shfl_down(c[r].x, s, 64);
+            oc[r].y = UP ? __shfl_up(c[r].y, s, 64) : __shfl_down(c[r].y, s, 64);
+        } else {
+            oc[r] = ss_c(0.0, 0.0);
+        }
+    }
+}
+
+struct ScanShared {
+    long long next_sim;   // first simulation (scan order) not yet marked
+    int count;
+    int pmin;
+};
+
+// start_sim[i] = the simulation starting at tile row r0 + i, or -1, for the
+// rows [r0, r1); sh.next_sim moves past every simulation starting before r1.
+// Returns whether any row of the tile starts a simulation.
+template <bool REV>
+__device__ bool scan_mark_starts(const ScanArgs& a, long long r0, long long r1, int* start_sim, ScanShared& sh)
+{
+    const int t = threadIdx.x;
+    start_sim[t] = -1;
+    __syncthreads();
+    if (!a.offsets) {
+        if (r0 == 0 && t == 0) start_sim[0] = 0;
+        __syncthreads();
+        return r0 == 0;
+    }
+    int any = 0;
+#pragma unroll 1
+    for (;;) {
+        const long long i = sh.next_sim + t;
+        int hit = 0;
+        if (i < a.n_sims) {
+            const long long ls = scan_start<REV>(a, i);
+            if (ls < r1) {
+                hit = 1;
+                const long long s = scan_sim<REV>(a, i);
+                if (a.offsets[s + 1] > a.offsets[s]) {
+                    start_sim[ls - r0] = (int)s;
+                    any = 1;
+                }
+            }
+        }
+        const int cnt = __syncthreads_count(hit);
+        if (t == 0) sh.next_sim += cnt;
+        __syncthreads();
+        if (cnt < kScanThreads) break;
+    }
+    return __syncthreads_or(any);
+}
+
+// This thread's row map (identity past r1).
+template <int D, bool REV, bool AFF>
+__device__ __forceinline__ void scan_row_map(const ScanArgs& a, long long r, long long r1, int s, ss_cd m[D][D],
+                                             ss_cd c[D])
+{
+    set_zero_vec<D>(c);
+    if (r >= r1) {
+        ss_set_identity<D>(m);
+    } else if (AFF && s >= 0) {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int k = 0; k < D; ++k) m[i][k] = ss_c(0.0, 0.0);
+        scan_init_vec<D>(a, s, c);
+    } else {
+        scan_row_mat<D, REV>(a, r, m);
+    }
+}
+
+// Pass 1 on one tile: (gm, gc) <- (tile map) o (gm, gc), kept in LDS.
+template <int D, bool REV, bool AFF>
+__device__ void scan_tile_reduce(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
+                                 ss_cd (&s_tm)[4][D][D], ss_cd (&s_tc)[4][D], ss_cd (&gm)[D][D], ss_cd (&gc)[D])
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    ss_cd xm[D][D], xc[D];
+    scan_row_map<D, REV, AFF>(a, r0 + t, r1, start_sim[t], xm, xc);
+    // x_l <- x_{l+s} o x_l: lane 0 ends with the wave's map (later rows on the left)
+#pragma unroll 1
+    for (int s = 1; s < 64; s <<= 1) {
+        ss_cd om[D][D], oc[D];
+        shfl_aff<D, AFF, false>(xm, xc, om, oc, s);
+        if ((lane & (2 * s - 1)) == 0) aff_after<D, AFF>(om, oc, xm, xc);
+    }
+    if (lane == 0) {
+        ss_copy<D>(xm, s_tm[w]);
+#pragma unroll
+        for (int r = 0; r < D; ++r) s_tc[w][r] = xc[r];
+    }
+    __syncthreads();
+    if (t == 0) {
+        // the chunk's map may already be affine: compose in the affine form
+        ss_cd zc[D];
+        set_zero_vec<D>(zc);
+#pragma unroll 1
+        for (int ww = 0; ww < 4; ++ww) aff_after<D, true>(s_tm[ww], AFF ? s_tc[ww] : zc, gm, gc);
+    }
+    __syncthreads();
+}
+
+template <int D, bool REV>
+__global__ void __launch_bounds__(kScanThreads) scan_reduce(ScanArgs a)
+{
+    __shared__ ScanShared sh;
+    __shared__ int start_sim[kScanThreads];
+    __shared__ ss_cd s_tm[4][D][D];
+    __shared__ ss_cd s_tc[4][D];
+    __shared__ ss_cd gm[D][D];
+    __shared__ ss_cd gc[D];
+    const int t = threadIdx.x;
+    const long long j = blockIdx.x;
+    const long long c0 = j * a.chunk_rows;
+    const long long c1 = c0 + a.chunk_rows < a.total ? c0 + a.chunk_rows : a.total;
+    if (t == 0) {
+        if (a.offsets) sh.next_sim = scan_first_sim<REV>(a, c0);
+        ss_set_identity<D>(gm);
+        set_zero_vec<D>(gc);
+    }
+    bool constant = false;
+#pragma unroll 1
+    for (long long r0 = c0; r0 < c1; r0 += kScanThreads) {
+        const long long r1 = r0 + kScanThreads < c1 ? r0 + kScanThreads : c1;
+        if (scan_mark_starts<REV>(a, r0, r1, start_sim, sh)) {
+            scan_tile_reduce<D, REV, true>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
+            constant = true;
+        } else {
+            scan_tile_reduce<D, REV, false>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
+        }
+    }
+    if (t == 0) {
+        double* p = a.agg + j * scan_agg_stride(D);
 #pragma unroll
         for (int r = 0; r < D; ++r)
 #pragma unroll
-            for (int c = 0; c < D; ++c) m[r][c] = ss_conj(t[c][r]);
+            for (int c = 0; c < D; ++c) {
+                p[2 * (r * D + c)] = gm[r][c].x;
+                p[2 * (r * D + c) + 1] = gm[r][c].y;
+            }
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            p[2 * D * D + 2 * r] = gc[r].x;
+            p[2 * D * D + 2 * r + 1] = gc[r].y;
+        }
+        p[2 * D * D + 2 * D] = constant ? 1.0 : 0.0;
     }
 }
 
-// One workgroup per simulation of a batch: the same three phases within it
-// (lane chunks, workgroup scan, walk).
-template <int D, bool REV>
-__global__ void __launch_bounds__(kScanBlock) batch_scan(const double* init, const double* u, double* out,
-                                                           const long long* offsets, double* spin)
+// Incoming state of chunk j: (map of chunk j-1) o ... applied back to the
+// nearest chunk whose map is constant (one in which a simulation starts).
+template <int D>
+__device__ void scan_incoming(const ScanArgs& a, long long j, ss_cd vin[D], ScanShared& sh,
+                              ss_cd (&s_m)[4][D][D], ss_cd (&s_v)[D])
 {
-    const long long sim = blockIdx.x;
-    const int lane = threadIdx.x;
-    const long long o0 = offsets[sim];
-    const long long n = offsets[sim + 1] - o0;
-    if (n <= 0) return;
-    const long long chunk = (n + kScanBlock - 1) / kScanBlock;
-    const long long k0 = lane * chunk;
-    const long long kend_m = k0 + chunk < n - 1 ? k0 + chunk : n - 1;
-    const long long kend_s = k0 + chunk < n ? k0 + chunk : n;
-    ss_cd x[D][D];
-    ss_set_identity<D>(x);
-    for (long long k = k0; k < kend_m; ++k) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    constexpr int kStride = scan_agg_stride(D);
+    ss_cd acc[D][D];
+    ss_set_identity<D>(acc);
+#pragma unroll 1
+    for (long long k = j - 1;; k -= kScanThreads) {
+        const long long idx = k - t;
         ss_cd m[D][D];
-        scan_step_mat<D, REV>(u, o0, n, k, m);
-        ss_left_multiply<D>(m, x);
-    }
-    ss_cd ex[D][D], total[D][D];
-    block_scan<D>(x, ex, total);
-    ss_cd p[D], v[D];
-    load_vec<D>(init, sim, p);
-    ss_matvec<D>(ex, p, v);
-    for (long long k = k0; k < kend_s; ++k) {
-        store_vec<D>(out, REV ? o0 + n - 1 - k : o0 + k, v);
-        if (spin) store_spin<D>(v, spin, REV ? o0 + n - 1 - k : o0 + k);
-        if (k + 1 < kend_s) {
-            ss_cd m[D][D], t[D];
-            scan_step_mat<D, REV>(u, o0, n, k, m);
-            ss_matvec<D>(m, v, t);
-#pragma unroll
-            for (int r = 0; r < D; ++r) v[r] = t[r];
+        ss_set_identity<D>(m);
+        bool constant = true;  // (no chunk before 0; chunk 0 is constant anyway)
+        if (idx >= 0) {
+            const double* p = a.agg + idx * kStride;
+            constant = p[2 * D * D + 2 * D] != 0.0;
         }
+        if (t == 0) sh.pmin = kScanThreads;
+        __syncthreads();
+        if (constant) atomicMin(&sh.pmin, t);
+        __syncthreads();
+        const int pm = sh.pmin;
+        if (t < pm) load_mat_at<D>(a.agg + idx * kStride, m);
+        // ordered product m_0 m_1 ... m_{pm-1}, chunk k on the left
+#pragma unroll 1
+        for (int s = 1; s < 64; s <<= 1) {
+            ss_cd o[D][D];
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    o[r][c].x = __shfl_down(m[r][c].x, s, 64);
+                    o[r][c].y = __shfl_down(m[r][c].y, s, 64);
+                }
+            if ((lane & (2 * s - 1)) == 0 && lane + s < 64) ss_right_multiply<D>(m, o);
+        }
+        if (lane == 0) ss_copy<D>(m, s_m[w]);
+        if (pm < kScanThreads && t == pm) {
+            if (idx >= 0) {
+                load_vec_c<D>(a.agg + idx * kStride + 2 * D * D, s_v);
+            } else {
+#pragma unroll
+                for (int r = 0; r < D; ++r) s_v[r] = ss_c(0.0, 0.0);
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int ww = 0; ww < 4; ++ww) ss_right_multiply<D>(acc, s_m[ww]);
+        const bool done = pm < kScanThreads;
+        if (done) ss_matvec<D>(acc, s_v, vin);
+        __syncthreads();
+        if (done) break;
+    }
+}
+
+// Pass 2 on one tile: states of rows [r0, r1) from the incoming state vin
+// (updated to the tile's last state).
+template <int D, bool REV, bool AFF>
+__device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
+                                ss_cd (&s_tm)[4][D][D], ss_cd (&s_tc)[4][D], ss_cd (&s_v)[D], ss_cd vin[D])
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const long long r = r0 + t;
+    ss_cd qm[D][D], qc[D];
+    scan_row_map<D, REV, AFF>(a, r, r1, start_sim[t], qm, qc);
+    // inclusive scan over the wave: q_l <- q_l o q_{l-s}
+#pragma unroll 1
+    for (int s = 1; s < 64; s <<= 1) {
+        ss_cd pm[D][D], pc[D];
+        shfl_aff<D, AFF, true>(qm, qc, pm, pc, s);
+        if (lane >= s) aff_before<D, AFF>(qm, qc, pm, pc);
+    }
+    if (lane == 63) {
+        ss_copy<D>(qm, s_tm[w]);
+#pragma unroll
+        for (int k = 0; k < D; ++k) s_tc[w][k] = qc[k];
+    }
+    __syncthreads();
+    {
+        ss_cd pm[D][D], pc[D];
+        ss_set_identity<D>(pm);
+        set_zero_vec<D>(pc);
+#pragma unroll 1
+        for (int ww = 0; ww < w; ++ww) aff_after<D, AFF>(s_tm[ww], s_tc[ww], pm, pc);
+        aff_before<D, AFF>(qm, qc, pm, pc);
+    }
+    ss_cd v[D];
+    ss_matvec<D>(qm, vin, v);
+    if (AFF) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] = ss_add(v[k], qc[k]);
+    }
+    if (r < r1) {
+        const long long o = REV ? a.total - 1 - r : r;
+        store_vec<D>(a.out, o, v);
+        if (a.spin) store_spin<D>(v, a.spin, o);
+        if (r == r1 - 1) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) s_v[k] = v[k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < D; ++k) vin[k] = s_v[k];
+    __syncthreads();
+}
+
+template <int D, bool REV>
+__global__ void __launch_bounds__(kScanThreads) scan_apply(ScanArgs a)
+{
+    __shared__ ScanShared sh;
+    __shared__ int start_sim[kScanThreads];
+    __shared__ ss_cd s_tm[4][D][D];
+    __shared__ ss_cd s_tc[4][D];
+    __shared__ ss_cd s_v[D];
+    const int t = threadIdx.x;
+    const long long j = blockIdx.x;
+    const long long c0 = j * a.chunk_rows;
+    const long long c1 = c0 + a.chunk_rows < a.total ? c0 + a.chunk_rows : a.total;
+    if (t == 0 && a.offsets) sh.next_sim = scan_first_sim<REV>(a, c0);
+    ss_cd vin[D];
+    set_zero_vec<D>(vin);
+    if (j > 0) scan_incoming<D>(a, j, vin, sh, s_tm, s_v);
+#pragma unroll 1
+    for (long long r0 = c0; r0 < c1; r0 += kScanThreads) {
+        const long long r1 = r0 + kScanThreads < c1 ? r0 + kScanThreads : c1;
+        if (scan_mark_starts<REV>(a, r0, r1, start_sim, sh))
+            scan_tile_apply<D, REV, true>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
+        else
+            scan_tile_apply<D, REV, false>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
     }
 }
 
@@ -397,20 +673,20 @@ __global__ void zero_counter(unsigned long long* p)
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct ScanLayout {
-    long long lanes, blocks;
-    size_t prefix_off, total_off, carry_off, bytes;
+    long long tiles, chunk_rows, chunks;
+    size_t bytes;
 };
 
+// Chunks of whole 256-row tiles, at most about 4096 of them; the workspace
+// holds one record (map and constant flag) per chunk.
 ScanLayout scan_layout(int64_t n, int dim)
 {
     ScanLayout l{};
-    l.lanes = (n + kScanChunk - 1) / kScanChunk;
-    l.blocks = (l.lanes + kScanBlock - 1) / kScanBlock;
-    const size_t mat = size_t(dim) * dim * 16;
-    l.prefix_off = 0;
-    l.total_off = align256(size_t(l.blocks) * kScanBlock * mat);
-    l.carry_off = l.total_off + align256(size_t(l.blocks) * mat);
-    l.bytes = l.carry_off + align256(size_t(l.blocks) * dim * 16);
+    l.tiles = (n + 255) / 256;
+    const long long per = std::max(1LL, (l.tiles + 4095) / 4096);
+    l.chunk_rows = 256 * per;
+    l.chunks = (n + l.chunk_rows - 1) / l.chunk_rows;
+    l.bytes = align256(size_t(std::max(1LL, l.chunks)) * scan_agg_stride(dim) * 8);
     return l;
 }
 
@@ -748,76 +1024,99 @@ size_t ss_state_scan_workspace(int64_t n, int dim)
     return scan_layout(n, dim).bytes;
 }
 
+namespace {
+
+// Fill the scan arguments from a workspace and launch both passes.
+int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, void* workspace, size_t workspace_bytes,
+                hipStream_t s)
+{
+    const ScanLayout l = scan_layout(a.total, dim);
+    if (!workspace) return fail("%s: null workspace", who);
+    if (workspace_bytes < l.bytes) return fail("%s: workspace of %zu bytes, need %zu", who, workspace_bytes, l.bytes);
+    if (l.chunks > 2147483647LL) return fail("%s: too many rows", who);
+    a.agg = static_cast<double*>(workspace);
+    a.chunk_rows = l.chunk_rows;
+    a.n_chunks = l.chunks;
+    const dim3 g(unsigned(l.chunks)), b(kScanThreads);
+    if (dim == 2 && !reverse) {
+        hipLaunchKernelGGL((scan_reduce<2, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((scan_apply<2, false>), g, b, 0, s, a);
+    } else if (dim == 3 && !reverse) {
+        hipLaunchKernelGGL((scan_reduce<3, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((scan_apply<3, false>), g, b, 0, s, a);
+    } else if (dim == 2) {
+        hipLaunchKernelGGL((scan_reduce<2, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((scan_apply<2, true>), g, b, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((scan_reduce<3, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((scan_apply<3, true>), g, b, 0, s, a);
+    }
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
 int ss_state_scan(const double* psi0, const double* u, double* state, double* spin, int64_t n, int dim,
                   void* workspace, size_t workspace_bytes, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_state_scan: dim must be 2 or 3");
     if (n < 0) return fail("ss_state_scan: negative n");
     if (n == 0) return 0;
-    if (!psi0 || !u || !state || !workspace) return fail("ss_state_scan: null buffer");
-    const ScanLayout l = scan_layout(n, dim);
-    if (workspace_bytes < l.bytes)
-        return fail("ss_state_scan: workspace of %zu bytes, need %zu", workspace_bytes, l.bytes);
-    if (l.blocks > 2147483647LL) return fail("ss_state_scan: too many rows");
-    char* ws = static_cast<char*>(workspace);
-    double* prefix = reinterpret_cast<double*>(ws + l.prefix_off);
-    double* total = reinterpret_cast<double*>(ws + l.total_off);
-    double* carry = reinterpret_cast<double*>(ws + l.carry_off);
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const ss_cd p0 = ss_c(psi0[0], psi0[1]), p1 = ss_c(psi0[2], psi0[3]);
-    const ss_cd p2 = dim == 3 ? ss_c(psi0[4], psi0[5]) : ss_c(0, 0);
-    const dim3 grid(unsigned(l.blocks)), block(kScanBlock);
-    if (dim == 2) {
-        hipLaunchKernelGGL(scan_lanes<2>, grid, block, 0, s, u, (long long)n, prefix, total);
-        hipLaunchKernelGGL(scan_blocks<2>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
-        hipLaunchKernelGGL(scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
-    } else {
-        hipLaunchKernelGGL(scan_lanes<3>, grid, block, 0, s, u, (long long)n, prefix, total);
-        hipLaunchKernelGGL(scan_blocks<3>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
-        hipLaunchKernelGGL(scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
-    }
-    SS_HIP(hipGetLastError());
-    return 0;
+    if (!psi0 || !u || !state) return fail("ss_state_scan: null buffer");
+    ScanArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = u;
+    a.out = state;
+    a.spin = spin;
+    for (int i = 0; i < 2 * dim; ++i) a.psi0[i] = psi0[i];
+    a.total = n;
+    a.n_sims = 1;
+    return launch_scan("ss_state_scan", a, dim, false, workspace, workspace_bytes,
+                        static_cast<hipStream_t>(stream));
 }
 
-int ss_batch_state_scan(const double* psi0, const double* u, double* state, double* spin,
-                        const int64_t* offsets, int64_t n_sims, int dim, ss_stream stream)
+int ss_batch_state_scan(const double* psi0, const double* u, double* state, double* spin, const int64_t* offsets,
+                        int64_t n_sims, int64_t total, int dim, void* workspace, size_t workspace_bytes,
+                        ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_state_scan: dim must be 2 or 3");
-    if (n_sims < 0) return fail("ss_batch_state_scan: negative n_sims");
-    if (n_sims == 0) return 0;
+    if (n_sims < 0 || total < 0) return fail("ss_batch_state_scan: negative size");
+    if (n_sims == 0 || total == 0) return 0;
     if (!psi0 || !u || !state || !offsets) return fail("ss_batch_state_scan: null buffer");
     if (n_sims > 2147483647LL) return fail("ss_batch_state_scan: too many simulations");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const long long* off = reinterpret_cast<const long long*>(offsets);
-    if (dim == 2)
-        hipLaunchKernelGGL((batch_scan<2, false>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, psi0, u, state,
-                           off, spin);
-    else
-        hipLaunchKernelGGL((batch_scan<3, false>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, psi0, u, state,
-                           off, spin);
-    SS_HIP(hipGetLastError());
-    return 0;
+    ScanArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = u;
+    a.out = state;
+    a.spin = spin;
+    a.offsets = reinterpret_cast<const long long*>(offsets);
+    a.init = psi0;
+    a.total = total;
+    a.n_sims = n_sims;
+    return launch_scan("ss_batch_state_scan", a, dim, false, workspace, workspace_bytes,
+                        static_cast<hipStream_t>(stream));
 }
 
 int ss_batch_costate_scan(const double* lambda_final, const double* u, double* costate, const int64_t* offsets,
-                          int64_t n_sims, int dim, ss_stream stream)
+                          int64_t n_sims, int64_t total, int dim, void* workspace, size_t workspace_bytes,
+                          ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_costate_scan: dim must be 2 or 3");
-    if (n_sims < 0) return fail("ss_batch_costate_scan: negative n_sims");
-    if (n_sims == 0) return 0;
+    if (n_sims < 0 || total < 0) return fail("ss_batch_costate_scan: negative size");
+    if (n_sims == 0 || total == 0) return 0;
     if (!lambda_final || !u || !costate || !offsets) return fail("ss_batch_costate_scan: null buffer");
     if (n_sims > 2147483647LL) return fail("ss_batch_costate_scan: too many simulations");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const long long* off = reinterpret_cast<const long long*>(offsets);
-    if (dim == 2)
-        hipLaunchKernelGGL((batch_scan<2, true>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, lambda_final, u,
-                           costate, off, nullptr);
-    else
-        hipLaunchKernelGGL((batch_scan<3, true>), dim3(unsigned(n_sims)), dim3(kScanBlock), 0, s, lambda_final, u,
-                           costate, off, nullptr);
-    SS_HIP(hipGetLastError());
-    return 0;
+    ScanArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = u;
+    a.out = costate;
+    a.offsets = reinterpret_cast<const long long*>(offsets);
+    a.init = lambda_final;
+    a.total = total;
+    a.n_sims = n_sims;
+    return launch_scan("ss_batch_costate_scan", a, dim, true, workspace, workspace_bytes,
+                        static_cast<hipStream_t>(stream));
 }
 
 int ss_batch_final_state(const double* psi0, const double* u, const double* state, const int64_t* offsets,

@@ -220,13 +220,14 @@ def test_gaussian_pulse_known_answer():
 
 
 def test_state_scan_random_unitaries():
-    """ss_state_scan vs the sequential product on sizes around the chunk/block edges."""
+    """ss_state_scan vs the sequential product on sizes around tile edges and
+    look-backs over more than 256 tiles."""
     ss = _ss()
     import torch
 
     rng = np.random.default_rng(7)
     for dim in (2, 3):
-        for n in (1, 2, 7, 8, 9, 2047, 2048, 2049, 2050, 4097, 300001):
+        for n in (1, 2, 7, 8, 9, 255, 256, 257, 2047, 2048, 2049, 4097, 65536, 65793, 300001):
             print(f"case: state scan dim={dim} n={n}", flush=True)
             a = rng.normal(size=(n, dim, dim)) + 1j * rng.normal(size=(n, dim, dim))
             q, _ = np.linalg.qr(a)

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == 3
+    assert lib.ss_abi_version() == 4
     assert lib.ss_source_hash() != 0
 
 
