@@ -627,15 +627,51 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
     time_out[row] = tc;
 }
 
-// One simulation: get_time_evolution, spinsim/__init__.py:854-896.  Wave w
-// takes items w, w + waves, w + 2 waves, ... of the phase table; within an
-// item, group gi of `lanes` lanes integrates coarse row
-// phase_row + (item - first item of the phase) * (64 / lanes) + gi.  Phases
-// with fewer lanes per row (long items) come first and the finest last, so
-// every wave ends within about one fine step of the others.
-extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te_args a)
+// One item of a single-simulation launch (get_time_evolution,
+// spinsim/__init__.py:854-896): in phase p, whose items start at item number
+// `first`, group gi of `lanes` lanes integrates coarse row
+// phase_row[p] + (item - first) * (64 / lanes) + gi.
+static __device__ __forceinline__ void ss_te_item(const ss_te_args& a, long long item, int p, long long first,
+                                                  const ss_ctx* ctx)
 {
     const int lane = threadIdx.x & 63;
+    const int lanes = a.phase_lanes[p];
+    const int groups = 64 / lanes;
+    const int gi = lane / lanes;
+    const int rank = lane - gi * lanes;
+    const long long end = p + 1 < a.n_phases ? a.phase_row[p + 1] : a.n_coarse;
+    const long long row = a.phase_row[p] + (item - first) * groups + gi;
+    const bool active = gi < groups && row < end;
+    const int base = a.n_fine / lanes;
+    const int rem = a.n_fine - base * lanes;
+    const int j0 = rank * base + (rank < rem ? rank : rem);
+    const int count = active ? base + (rank < rem ? 1 : 0) : 0;
+    const double tc = ss_coarse_time(a.t0, a.dt_out, row);
+    const double rot = active ? ss_rotating_wave(tc, a.dt_out, a.sweep, ctx) : 0.0;
+    ss_cd u[SS_D][SS_D];
+    ss_integrate(u, j0, count, tc, rot, a.dt, a.off_a, a.off_b, a.sweep, ctx);
+    ss_group_reduce(u, rank, lanes);
+    if (active && rank == 0) ss_store(u, rot, a.dt_out, tc, row, a.time_out, a.u_out);
+}
+
+// One simulation, one item per wave (a single phase: phase_lanes[0] lanes per
+// coarse step for every row).
+extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te_args a)
+{
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= a.items) return;  // whole waves only
+    ss_ctx ctx;
+    ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
+    ctx.sim = 0;
+    ss_te_item(a, wave, 0, 0, &ctx);
+}
+
+// One simulation on the resident waves: wave w takes items w, w + waves,
+// w + 2 waves, ... of the phase table.  Phases with fewer lanes per row (long
+// items) come first and the finest last, so every wave ends within about one
+// fine step of the others.
+extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_phased_kernel(ss_te_args a)
+{
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     ss_ctx ctx;
     ctx.tables = reinterpret_cast<const ss_table*>(a.tables);
@@ -648,23 +684,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te
             first += a.phase_items[p];
             ++p;
         }
-        const int lanes = a.phase_lanes[p];
-        const int groups = 64 / lanes;
-        const int gi = lane / lanes;
-        const int rank = lane - gi * lanes;
-        const long long end = p + 1 < a.n_phases ? a.phase_row[p + 1] : a.n_coarse;
-        const long long row = a.phase_row[p] + (item - first) * groups + gi;
-        const bool active = gi < groups && row < end;
-        const int base = a.n_fine / lanes;
-        const int rem = a.n_fine - base * lanes;
-        const int j0 = rank * base + (rank < rem ? rank : rem);
-        const int count = active ? base + (rank < rem ? 1 : 0) : 0;
-        const double tc = ss_coarse_time(a.t0, a.dt_out, row);
-        const double rot = active ? ss_rotating_wave(tc, a.dt_out, a.sweep, &ctx) : 0.0;
-        ss_cd u[SS_D][SS_D];
-        ss_integrate(u, j0, count, tc, rot, a.dt, a.off_a, a.off_b, a.sweep, &ctx);
-        ss_group_reduce(u, rank, lanes);
-        if (active && rank == 0) ss_store(u, rot, a.dt_out, tc, row, a.time_out, a.u_out);
+        ss_te_item(a, item, p, first, &ctx);
     }
 }
 

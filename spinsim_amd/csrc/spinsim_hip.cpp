@@ -65,6 +65,7 @@ uint64_t fnv1a(const char* s, uint64_t h)
 struct ss_module {
     hipModule_t module = nullptr;
     hipFunction_t single = nullptr;
+    hipFunction_t phased = nullptr;
     hipFunction_t batch = nullptr;
     hipFunction_t adjoint = nullptr;
     int dim = 2;
@@ -72,7 +73,7 @@ struct ss_module {
     int expo = 0;
     int rotating = 1;
     int device = 0;
-    long long resident_waves = 0;  // single-simulation kernel: waves resident at once
+    long long resident_waves = 0;  // phased kernel: waves resident at once
     long long simds = 0;           // SIMDs of the device (4 per CU)
 };
 
@@ -157,7 +158,146 @@ __global__ void __launch_bounds__(256) spin_kernel(const double* state, double* 
     store_spin<D>(s, spin, i);
 }
 
-// ---- state scan: get_state (:1087-1111) as a two-pass chunked scan --------
+// ---- state scan of one simulation: three passes ---------------------------
+// The latency-bound case (1e5 rows take ~30 us): per-lane prefix products
+// within 256-lane workgroups, one workgroup scanning the block totals into
+// block carries, and a pass applying prefix x carry to psi0 per row.
+
+constexpr int kSingleScanChunk = 1;   // matrices per lane
+constexpr int kSingleScanBlock = 256;
+
+template <int D>
+__device__ __forceinline__ void shfl_up_mat(const ss_cd m[D][D], ss_cd out[D][D], int s)
+{
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            out[r][c].x = __shfl_up(m[r][c].x, s, 64);
+            out[r][c].y = __shfl_up(m[r][c].y, s, 64);
+        }
+}
+
+// Inclusive scan over the 64 lanes of a wave of x_l (later lanes on the
+// left): q_l = x_l x_{l-1} ... x_0.  Then ex = q_{l-1} (identity on lane 0).
+template <int D>
+__device__ __forceinline__ void wave_scan(ss_cd q[D][D], ss_cd ex[D][D], int lane)
+{
+#pragma unroll 1
+    for (int s = 1; s < 64; s <<= 1) {
+        ss_cd p[D][D];
+        shfl_up_mat<D>(q, p, s);
+        if (lane >= s) ss_right_multiply<D>(q, p);
+    }
+    shfl_up_mat<D>(q, ex, 1);
+    if (lane == 0) ss_set_identity<D>(ex);
+}
+
+// Workgroup-wide exclusive scan (256 lanes = 4 waves).  On return `ex` is
+// the product of the x of all earlier lanes of the workgroup and `total`
+// (valid on every lane) the product over the whole workgroup.
+template <int D>
+__device__ __forceinline__ void block_scan(const ss_cd x[D][D], ss_cd ex[D][D], ss_cd total[D][D])
+{
+    __shared__ ss_cd wave_tot[kSingleScanBlock / 64][D][D];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    ss_cd q[D][D];
+    ss_copy<D>(x, q);
+    ss_cd wex[D][D];
+    wave_scan<D>(q, wex, lane);
+    if (lane == 63) ss_copy<D>(q, wave_tot[wave]);
+    __syncthreads();
+    ss_cd pre[D][D];
+    ss_set_identity<D>(pre);
+    for (int w = 0; w < wave; ++w) ss_left_multiply<D>(wave_tot[w], pre);
+    ss_matmul<D>(wex, pre, ex);
+    ss_set_identity<D>(total);
+    for (int w = 0; w < kSingleScanBlock / 64; ++w) ss_left_multiply<D>(wave_tot[w], total);
+}
+
+// Single simulation, pass 1: lane L covers states [L*C, L*C + C) and the matrices it applies,
+// [L*C, min(L*C + C, n - 1)).  Store each lane's exclusive in-block prefix and
+// each block's total.
+template <int D>
+__global__ void __launch_bounds__(kSingleScanBlock) single_scan_lanes(const double* u, long long n, double* lane_prefix,
+                                                           double* block_total)
+{
+    const long long lane_id = (long long)blockIdx.x * kSingleScanBlock + threadIdx.x;
+    const long long k0 = lane_id * kSingleScanChunk;
+    const long long kend = k0 + kSingleScanChunk < n - 1 ? k0 + kSingleScanChunk : n - 1;
+    ss_cd x[D][D];
+    ss_set_identity<D>(x);
+    for (long long k = k0; k < kend; ++k) {
+        ss_cd m[D][D];
+        load_mat<D>(u, k, m);
+        ss_left_multiply<D>(m, x);
+    }
+    ss_cd ex[D][D], total[D][D];
+    block_scan<D>(x, ex, total);
+    store_mat<D>(lane_prefix, lane_id, ex);
+    if (threadIdx.x == 0) store_mat<D>(block_total, blockIdx.x, total);
+}
+
+// Single simulation, pass 2 (one workgroup): carry-in state of every block,
+// v_b = (B_{b-1} ... B_0) psi0.
+template <int D>
+__global__ void __launch_bounds__(kSingleScanBlock) single_scan_blocks(const double* block_total, long long nb,
+                                                            ss_cd p0, ss_cd p1, ss_cd p2, double* carry)
+{
+    const long long per = (nb + kSingleScanBlock - 1) / kSingleScanBlock;
+    const long long b0 = per * threadIdx.x;
+    const long long b1 = b0 + per < nb ? b0 + per : nb;
+    ss_cd x[D][D];
+    ss_set_identity<D>(x);
+    for (long long b = b0; b < b1; ++b) {
+        ss_cd m[D][D];
+        load_mat<D>(block_total, b, m);
+        ss_left_multiply<D>(m, x);
+    }
+    ss_cd ex[D][D], total[D][D];
+    block_scan<D>(x, ex, total);
+    ss_cd psi[3] = {p0, p1, p2};
+    ss_cd v[D];
+    ss_matvec<D>(ex, psi, v);
+    for (long long b = b0; b < b1; ++b) {
+        store_vec<D>(carry, b, v);
+        ss_cd m[D][D], t[D];
+        load_mat<D>(block_total, b, m);
+        ss_matvec<D>(m, v, t);
+#pragma unroll
+        for (int r = 0; r < D; ++r) v[r] = t[r];
+    }
+}
+
+// Single simulation, pass 3: each lane starts from (its in-block prefix) * (block carry) and
+// walks its chunk, writing the states.
+template <int D>
+__global__ void __launch_bounds__(kSingleScanBlock) single_scan_apply(const double* u, long long n, const double* lane_prefix,
+                                                           const double* carry, double* state, double* spin)
+{
+    const long long lane_id = (long long)blockIdx.x * kSingleScanBlock + threadIdx.x;
+    const long long k0 = lane_id * kSingleScanChunk;
+    if (k0 >= n) return;
+    const long long kend = k0 + kSingleScanChunk < n ? k0 + kSingleScanChunk : n;
+    ss_cd pre[D][D], c[D], v[D];
+    load_mat<D>(lane_prefix, lane_id, pre);
+    load_vec<D>(carry, blockIdx.x, c);
+    ss_matvec<D>(pre, c, v);
+    for (long long k = k0; k < kend; ++k) {
+        store_vec<D>(state, k, v);
+        if (spin) store_spin<D>(v, spin, k);
+        if (k + 1 < kend) {
+            ss_cd m[D][D], t[D];
+            load_mat<D>(u, k, m);
+            ss_matvec<D>(m, v, t);
+#pragma unroll
+            for (int r = 0; r < D; ++r) v[r] = t[r];
+        }
+    }
+}
+
+// ---- state scan: get_state (:1087-1111) batches and costates, two passes -----
 //
 // Rows are taken in scan order (forward for states, backward for costates).
 // Row r maps the previous row's state to its own: v -> U v with U the matrix
@@ -357,11 +497,12 @@ struct ScanShared {
 // start_sim[i] = the simulation starting at tile row r0 + i, or -1, for the
 // rows [r0, r1); sh.next_sim moves past every simulation starting before r1.
 // Returns whether any row of the tile starts a simulation.
-template <bool REV>
+template <bool REV, int R>
 __device__ bool scan_mark_starts(const ScanArgs& a, long long r0, long long r1, int* start_sim, ScanShared& sh)
 {
     const int t = threadIdx.x;
-    start_sim[t] = -1;
+#pragma unroll
+    for (int k = 0; k < R; ++k) start_sim[k * kScanThreads + t] = -1;
     __syncthreads();
     if (!a.offsets) {
         if (r0 == 0 && t == 0) start_sim[0] = 0;
@@ -411,14 +552,29 @@ __device__ __forceinline__ void scan_row_map(const ScanArgs& a, long long r, lon
     }
 }
 
+// Composite map of this thread's R consecutive rows, T_{R-1} o ... o T_0.
+template <int D, bool REV, int R, bool AFF>
+__device__ __forceinline__ void scan_thread_map(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
+                                                ss_cd xm[D][D], ss_cd xc[D])
+{
+    const int t = threadIdx.x;
+    scan_row_map<D, REV, AFF>(a, r0 + (long long)t * R, r1, start_sim[t * R], xm, xc);
+#pragma unroll 1
+    for (int k = 1; k < R; ++k) {
+        ss_cd m[D][D], c[D];
+        scan_row_map<D, REV, AFF>(a, r0 + (long long)t * R + k, r1, start_sim[t * R + k], m, c);
+        aff_after<D, AFF>(m, c, xm, xc);
+    }
+}
+
 // Pass 1 on one tile: (gm, gc) <- (tile map) o (gm, gc), kept in LDS.
-template <int D, bool REV, bool AFF>
+template <int D, bool REV, int R, bool AFF>
 __device__ void scan_tile_reduce(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
                                  ss_cd (&s_tm)[4][D][D], ss_cd (&s_tc)[4][D], ss_cd (&gm)[D][D], ss_cd (&gc)[D])
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     ss_cd xm[D][D], xc[D];
-    scan_row_map<D, REV, AFF>(a, r0 + t, r1, start_sim[t], xm, xc);
+    scan_thread_map<D, REV, R, AFF>(a, r0, r1, start_sim, xm, xc);
     // x_l <- x_{l+s} o x_l: lane 0 ends with the wave's map (later rows on the left)
 #pragma unroll 1
     for (int s = 1; s < 64; s <<= 1) {
@@ -442,11 +598,11 @@ __device__ void scan_tile_reduce(const ScanArgs& a, long long r0, long long r1, 
     __syncthreads();
 }
 
-template <int D, bool REV>
-__global__ void __launch_bounds__(kScanThreads) scan_reduce(ScanArgs a)
+template <int D, bool REV, int R>
+__global__ void __launch_bounds__(kScanThreads) batch_scan_reduce(ScanArgs a)
 {
     __shared__ ScanShared sh;
-    __shared__ int start_sim[kScanThreads];
+    __shared__ int start_sim[kScanThreads * R];
     __shared__ ss_cd s_tm[4][D][D];
     __shared__ ss_cd s_tc[4][D];
     __shared__ ss_cd gm[D][D];
@@ -462,13 +618,13 @@ __global__ void __launch_bounds__(kScanThreads) scan_reduce(ScanArgs a)
     }
     bool constant = false;
 #pragma unroll 1
-    for (long long r0 = c0; r0 < c1; r0 += kScanThreads) {
-        const long long r1 = r0 + kScanThreads < c1 ? r0 + kScanThreads : c1;
-        if (scan_mark_starts<REV>(a, r0, r1, start_sim, sh)) {
-            scan_tile_reduce<D, REV, true>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
+    for (long long r0 = c0; r0 < c1; r0 += kScanThreads * R) {
+        const long long r1 = r0 + kScanThreads * R < c1 ? r0 + kScanThreads * R : c1;
+        if (scan_mark_starts<REV, R>(a, r0, r1, start_sim, sh)) {
+            scan_tile_reduce<D, REV, R, true>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
             constant = true;
         } else {
-            scan_tile_reduce<D, REV, false>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
+            scan_tile_reduce<D, REV, R, false>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
         }
     }
     if (t == 0) {
@@ -549,14 +705,13 @@ __device__ void scan_incoming(const ScanArgs& a, long long j, ss_cd vin[D], Scan
 
 // Pass 2 on one tile: states of rows [r0, r1) from the incoming state vin
 // (updated to the tile's last state).
-template <int D, bool REV, bool AFF>
+template <int D, bool REV, int R, bool AFF>
 __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
                                 ss_cd (&s_tm)[4][D][D], ss_cd (&s_tc)[4][D], ss_cd (&s_v)[D], ss_cd vin[D])
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const long long r = r0 + t;
     ss_cd qm[D][D], qc[D];
-    scan_row_map<D, REV, AFF>(a, r, r1, start_sim[t], qm, qc);
+    scan_thread_map<D, REV, R, AFF>(a, r0, r1, start_sim, qm, qc);
     // inclusive scan over the wave: q_l <- q_l o q_{l-s}
 #pragma unroll 1
     for (int s = 1; s < 64; s <<= 1) {
@@ -568,6 +723,18 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
         ss_copy<D>(qm, s_tm[w]);
 #pragma unroll
         for (int k = 0; k < D; ++k) s_tc[w][k] = qc[k];
+    }
+    if (R > 1) {
+        // exclusive: the rows of earlier lanes of the wave
+        ss_cd em[D][D], ec[D];
+        shfl_aff<D, AFF, true>(qm, qc, em, ec, 1);
+        if (lane == 0) {
+            ss_set_identity<D>(em);
+            set_zero_vec<D>(ec);
+        }
+        ss_copy<D>(em, qm);
+#pragma unroll
+        for (int k = 0; k < D; ++k) qc[k] = ec[k];
     }
     __syncthreads();
     {
@@ -584,13 +751,24 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
 #pragma unroll
         for (int k = 0; k < D; ++k) v[k] = ss_add(v[k], qc[k]);
     }
-    if (r < r1) {
+#pragma unroll 1
+    for (int k = 0; k < R; ++k) {
+        const long long r = r0 + (long long)t * R + k;
+        if (r >= r1) break;
+        if (R > 1) {
+            // this row's map applied to the previous row's state
+            ss_cd m[D][D], c[D], nv[D];
+            scan_row_map<D, REV, AFF>(a, r, r1, start_sim[t * R + k], m, c);
+            ss_matvec<D>(m, v, nv);
+#pragma unroll
+            for (int i = 0; i < D; ++i) v[i] = AFF ? ss_add(nv[i], c[i]) : nv[i];
+        }
         const long long o = REV ? a.total - 1 - r : r;
         store_vec<D>(a.out, o, v);
         if (a.spin) store_spin<D>(v, a.spin, o);
         if (r == r1 - 1) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) s_v[k] = v[k];
+            for (int i = 0; i < D; ++i) s_v[i] = v[i];
         }
     }
     __syncthreads();
@@ -599,11 +777,11 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
     __syncthreads();
 }
 
-template <int D, bool REV>
-__global__ void __launch_bounds__(kScanThreads) scan_apply(ScanArgs a)
+template <int D, bool REV, int R>
+__global__ void __launch_bounds__(kScanThreads) batch_scan_apply(ScanArgs a)
 {
     __shared__ ScanShared sh;
-    __shared__ int start_sim[kScanThreads];
+    __shared__ int start_sim[kScanThreads * R];
     __shared__ ss_cd s_tm[4][D][D];
     __shared__ ss_cd s_tc[4][D];
     __shared__ ss_cd s_v[D];
@@ -616,12 +794,12 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply(ScanArgs a)
     set_zero_vec<D>(vin);
     if (j > 0) scan_incoming<D>(a, j, vin, sh, s_tm, s_v);
 #pragma unroll 1
-    for (long long r0 = c0; r0 < c1; r0 += kScanThreads) {
-        const long long r1 = r0 + kScanThreads < c1 ? r0 + kScanThreads : c1;
-        if (scan_mark_starts<REV>(a, r0, r1, start_sim, sh))
-            scan_tile_apply<D, REV, true>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
+    for (long long r0 = c0; r0 < c1; r0 += kScanThreads * R) {
+        const long long r1 = r0 + kScanThreads * R < c1 ? r0 + kScanThreads * R : c1;
+        if (scan_mark_starts<REV, R>(a, r0, r1, start_sim, sh))
+            scan_tile_apply<D, REV, R, true>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
         else
-            scan_tile_apply<D, REV, false>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
+            scan_tile_apply<D, REV, R, false>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
     }
 }
 
@@ -672,19 +850,41 @@ __global__ void zero_counter(unsigned long long* p)
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+struct SingleScanLayout {
+    long long lanes, blocks;
+    size_t prefix_off, total_off, carry_off, bytes;
+};
+
+SingleScanLayout single_scan_layout(int64_t n, int dim)
+{
+    SingleScanLayout l{};
+    l.lanes = (n + kSingleScanChunk - 1) / kSingleScanChunk;
+    l.blocks = (l.lanes + kSingleScanBlock - 1) / kSingleScanBlock;
+    const size_t mat = size_t(dim) * dim * 16;
+    l.prefix_off = 0;
+    l.total_off = align256(size_t(l.blocks) * kSingleScanBlock * mat);
+    l.carry_off = l.total_off + align256(size_t(l.blocks) * mat);
+    l.bytes = l.carry_off + align256(size_t(l.blocks) * dim * 16);
+    return l;
+}
+
 struct ScanLayout {
     long long tiles, chunk_rows, chunks;
     size_t bytes;
 };
 
-// Chunks of whole 256-row tiles, at most about 4096 of them; the workspace
-// holds one record (map and constant flag) per chunk.
+// Rows per thread of the state scan; a tile is 256 threads.
+constexpr int kScanRows = 4;
+
+// Chunks of whole tiles, at most about 2048 of them (8 workgroups per CU);
+// the workspace holds one record (map and constant flag) per chunk.
 ScanLayout scan_layout(int64_t n, int dim)
 {
     ScanLayout l{};
-    l.tiles = (n + 255) / 256;
-    const long long per = std::max(1LL, (l.tiles + 4095) / 4096);
-    l.chunk_rows = 256 * per;
+    const long long tile = 256LL * kScanRows;
+    l.tiles = (n + tile - 1) / tile;
+    const long long per = std::max(1LL, (l.tiles + 2047) / 2048);
+    l.chunk_rows = tile * per;
     l.chunks = (n + l.chunk_rows - 1) / l.chunk_rows;
     l.bytes = align256(size_t(std::max(1LL, l.chunks)) * scan_agg_stride(dim) * 8);
     return l;
@@ -800,10 +1000,11 @@ int ss_module_load(const void* code, size_t code_size, int dim, int method, int 
     hipError_t e = hipGetDevice(&m->device);
     if (e == hipSuccess) e = hipModuleLoadData(&m->module, code);
     if (e == hipSuccess) e = hipModuleGetFunction(&m->single, m->module, "ss_time_evolution_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&m->phased, m->module, "ss_time_evolution_phased_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&m->batch, m->module, "ss_batch_time_evolution_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&m->adjoint, m->module, "ss_batch_adjoint_kernel");
     int per_cu = 0, cus = 0;
-    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->single, 256, 0);
+    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->phased, 256, 0);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device);
     if (e != hipSuccess) {
         if (m->module) (void)hipModuleUnload(m->module);
@@ -961,7 +1162,7 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
     const long long blocks = (a.waves + 3) / 4;
     if (blocks > 2147483647LL) return fail("ss_time_evolution: grid too large");
     void* params[] = {&a};
-    SS_HIP(hipModuleLaunchKernel(m->single, unsigned(blocks), 1, 1, 256, 1, 1, 0,
+    SS_HIP(hipModuleLaunchKernel(lanes_per_step > 0 ? m->single : m->phased, unsigned(blocks), 1, 1, 256, 1, 1, 0,
                                  static_cast<hipStream_t>(stream), params, nullptr));
     return 0;
 }
@@ -1021,7 +1222,7 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
 size_t ss_state_scan_workspace(int64_t n, int dim)
 {
     if (n <= 0 || (dim != 2 && dim != 3)) return 0;
-    return scan_layout(n, dim).bytes;
+    return std::max(scan_layout(n, dim).bytes, single_scan_layout(n, dim).bytes);
 }
 
 namespace {
@@ -1038,18 +1239,26 @@ int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, void* worksp
     a.chunk_rows = l.chunk_rows;
     a.n_chunks = l.chunks;
     const dim3 g(unsigned(l.chunks)), b(kScanThreads);
+    // pass 2 with one row per thread once chunks span several tiles (large,
+    // bandwidth-bound scans: row-contiguous lanes store coalesced); with
+    // kScanRows rows per thread when each chunk is one tile (latency-bound)
+    const bool wide = l.chunk_rows > 256LL * kScanRows;
     if (dim == 2 && !reverse) {
-        hipLaunchKernelGGL((scan_reduce<2, false>), g, b, 0, s, a);
-        hipLaunchKernelGGL((scan_apply<2, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((batch_scan_reduce<2, false, kScanRows>), g, b, 0, s, a);
+        if (wide) hipLaunchKernelGGL((batch_scan_apply<2, false, 1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((batch_scan_apply<2, false, kScanRows>), g, b, 0, s, a);
     } else if (dim == 3 && !reverse) {
-        hipLaunchKernelGGL((scan_reduce<3, false>), g, b, 0, s, a);
-        hipLaunchKernelGGL((scan_apply<3, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((batch_scan_reduce<3, false, kScanRows>), g, b, 0, s, a);
+        if (wide) hipLaunchKernelGGL((batch_scan_apply<3, false, 1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((batch_scan_apply<3, false, kScanRows>), g, b, 0, s, a);
     } else if (dim == 2) {
-        hipLaunchKernelGGL((scan_reduce<2, true>), g, b, 0, s, a);
-        hipLaunchKernelGGL((scan_apply<2, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((batch_scan_reduce<2, true, kScanRows>), g, b, 0, s, a);
+        if (wide) hipLaunchKernelGGL((batch_scan_apply<2, true, 1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((batch_scan_apply<2, true, kScanRows>), g, b, 0, s, a);
     } else {
-        hipLaunchKernelGGL((scan_reduce<3, true>), g, b, 0, s, a);
-        hipLaunchKernelGGL((scan_apply<3, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((batch_scan_reduce<3, true, kScanRows>), g, b, 0, s, a);
+        if (wide) hipLaunchKernelGGL((batch_scan_apply<3, true, 1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((batch_scan_apply<3, true, kScanRows>), g, b, 0, s, a);
     }
     SS_HIP(hipGetLastError());
     return 0;
@@ -1063,17 +1272,30 @@ int ss_state_scan(const double* psi0, const double* u, double* state, double* sp
     if (dim != 2 && dim != 3) return fail("ss_state_scan: dim must be 2 or 3");
     if (n < 0) return fail("ss_state_scan: negative n");
     if (n == 0) return 0;
-    if (!psi0 || !u || !state) return fail("ss_state_scan: null buffer");
-    ScanArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.u = u;
-    a.out = state;
-    a.spin = spin;
-    for (int i = 0; i < 2 * dim; ++i) a.psi0[i] = psi0[i];
-    a.total = n;
-    a.n_sims = 1;
-    return launch_scan("ss_state_scan", a, dim, false, workspace, workspace_bytes,
-                        static_cast<hipStream_t>(stream));
+    if (!psi0 || !u || !state || !workspace) return fail("ss_state_scan: null buffer");
+    const SingleScanLayout l = single_scan_layout(n, dim);
+    if (workspace_bytes < l.bytes)
+        return fail("ss_state_scan: workspace of %zu bytes, need %zu", workspace_bytes, l.bytes);
+    if (l.blocks > 2147483647LL) return fail("ss_state_scan: too many rows");
+    char* ws = static_cast<char*>(workspace);
+    double* prefix = reinterpret_cast<double*>(ws + l.prefix_off);
+    double* total = reinterpret_cast<double*>(ws + l.total_off);
+    double* carry = reinterpret_cast<double*>(ws + l.carry_off);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const ss_cd p0 = ss_c(psi0[0], psi0[1]), p1 = ss_c(psi0[2], psi0[3]);
+    const ss_cd p2 = dim == 3 ? ss_c(psi0[4], psi0[5]) : ss_c(0, 0);
+    const dim3 grid(unsigned(l.blocks)), block(kSingleScanBlock);
+    if (dim == 2) {
+        hipLaunchKernelGGL(single_scan_lanes<2>, grid, block, 0, s, u, (long long)n, prefix, total);
+        hipLaunchKernelGGL(single_scan_blocks<2>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
+        hipLaunchKernelGGL(single_scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
+    } else {
+        hipLaunchKernelGGL(single_scan_lanes<3>, grid, block, 0, s, u, (long long)n, prefix, total);
+        hipLaunchKernelGGL(single_scan_blocks<3>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
+        hipLaunchKernelGGL(single_scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
+    }
+    SS_HIP(hipGetLastError());
+    return 0;
 }
 
 int ss_batch_state_scan(const double* psi0, const double* u, double* state, double* spin, const int64_t* offsets,

@@ -13,9 +13,11 @@ TOL = 1e-9
 
 
 def _unitaries(rng, n, d):
-    a = rng.normal(size=(n, d, d)) + 1j * rng.normal(size=(n, d, d))
-    q, _ = np.linalg.qr(a)
-    return q
+    """Random unitaries diag(exp(i phi)) (I - 2 v v^+ / |v|^2), built in bulk."""
+    v = rng.normal(size=(n, d)) + 1j * rng.normal(size=(n, d))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    h = np.eye(d)[None] - 2 * v[:, :, None] * v.conj()[:, None, :]
+    return np.exp(1j * rng.uniform(0, 2 * np.pi, (n, d)))[:, :, None] * h
 
 
 def _reference(u, offsets, init, reverse):
@@ -67,7 +69,7 @@ def _run(u, offsets, init, reverse, with_spin=False):
 CASES = {
     # name: simulation lengths
     "one_long": [70001],
-    "tile_edges": [255, 256, 257, 1, 511, 512, 513],
+    "tile_edges": [255, 256, 257, 1, 1023, 1024, 1025, 2047, 2049],
     "many_tiny": None,      # 3000 simulations of 0-1 rows: ~512 per tile
     "mixed": None,          # long and empty simulations interleaved
 }
@@ -106,20 +108,21 @@ def test_batch_scan_matches_recurrence(dim, name, reverse):
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("reverse", [False, True], ids=["state", "costate"])
 def test_batch_scan_multi_tile_chunks(dim, reverse):
-    """Over 2^20 rows, so each workgroup scans a chunk of several tiles; checked
-    against the C oracle's sequential product, one simulation at a time."""
+    """Over 2^21 rows, so each workgroup scans a chunk of several tiles;
+    checked against the C oracle's sequential product, one simulation at a
+    time."""
     import fields as F
     from oracle.c_oracle import COracle
 
     print(f"case: scan multi-tile chunks dim={dim} reverse={reverse}", flush=True)
     rng = np.random.default_rng(13)
-    n = rng.integers(0, 60000, 40)
+    n = rng.integers(0, 130000, 40)
     n[3] = 0
     n[11] = 1
     n[12] = 257
     offsets = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
     total = int(offsets[-1])
-    assert total > 1 << 20
+    assert total > 1 << 21
     u = _unitaries(rng, total, dim)
     init = rng.normal(size=(len(n), dim)) + 1j * rng.normal(size=(len(n), dim))
     got, spin = _run(u, offsets, init, reverse, with_spin=not reverse)
