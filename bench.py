@@ -288,6 +288,26 @@ def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_co
             "precision": precision}
 
 
+def fma_probe(torch, threads=256 * 4 * 8 * 64, iters=8192, reps=3):
+    """Sustained float64 FMA rate of the device (TFLOP/s): 8 waves per SIMD,
+    8 independent FMA chains per lane (ss_fma_probe), best of `reps`."""
+    from spinsim_amd import _native
+
+    out = torch.empty(threads, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    best = 0.0
+    for _ in range(reps + 1):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _native.check(_native.lib().ss_fma_probe(out.data_ptr(), threads, iters, stream.cuda_stream),
+                      "ss_fma_probe")
+        b.record(stream)
+        b.synchronize()
+        best = max(best, 64.0 * iters * threads / (a.elapsed_time(b) * 1e-3) / 1e12)
+    return best
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +338,7 @@ def main():
                                             lanes)
     n, nf, dt = main_run["coarse"], main_run["n_fine"], main_run["dt"]
     achieved = main_run["fine"] * FLOPS_PER_STEP / (main_run["kernel_ms"] * 1e-3) / 1e12
+    probe = fma_probe(torch)
 
     variants = {}
     if not args.no_variants:
@@ -367,6 +388,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC_BYTES},
+        "fp64_fma_probe": {"tflops": probe, "frac_of_published_peak": probe / FP64_PEAK_TFLOPS,
+                           "kernel_frac_of_probe": achieved / probe},
         "cpu_baseline": base,
         "variants": variants,
     }

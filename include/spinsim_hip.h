@@ -191,6 +191,12 @@ int ss_interp_eval(const ss_table *tables, int k, int sim, const double *t, doub
 /* Bloch vector of each of n states (dim complex each): spin is (n, 3). */
 int ss_spin(const double *state, double *spin, int64_t n, int dim, ss_stream stream);
 
+/* Measurement aid, no counterpart in the reference: `threads` lanes each run
+ * iters x 32 independent float64 FMAs (8 chains), writing one double each
+ * to out (device, `threads` doubles).  Flops = 64 * iters * threads; the
+ * launch's duration gives the sustained float64 FMA rate of the device. */
+int ss_fma_probe(double *out, int64_t threads, int iters, ss_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

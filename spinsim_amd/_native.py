@@ -53,6 +53,7 @@ SIGNATURES = {
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_vp]),
     "ss_interp_eval": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_i64, c_vp]),
     "ss_spin": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
+    "ss_fma_probe": (c_int, [c_vp, c_i64, c_int, c_vp]),
 }
 
 

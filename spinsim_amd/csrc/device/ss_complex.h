@@ -48,19 +48,22 @@ static __device__ __forceinline__ void ss_copy(const ss_cd a[D][D], ss_cd out[D]
         for (int c = 0; c < D; ++c) out[r][c] = a[r][c];
 }
 
-// out = a * b, summed in k order
+// out = a * b, each entry summed in k order.  The k loop is outermost so
+// consecutive fused multiply-adds update different entries: a single wave
+// then issues back to back without waiting on its own results.
 template <int D>
 static __device__ __forceinline__ void ss_matmul(const ss_cd a[D][D], const ss_cd b[D][D], ss_cd out[D][D])
 {
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
-        for (int c = 0; c < D; ++c) {
-            ss_cd acc = ss_mul(a[r][0], b[0][c]);
+        for (int c = 0; c < D; ++c) out[r][c] = ss_mul(a[r][0], b[0][c]);
 #pragma unroll
-            for (int k = 1; k < D; ++k) acc = ss_mac(acc, a[r][k], b[k][c]);
-            out[r][c] = acc;
-        }
+    for (int k = 1; k < D; ++k)
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) out[r][c] = ss_mac(out[r][c], a[r][k], b[k][c]);
 }
 
 // u <- e * u

@@ -299,32 +299,39 @@ static __device__ __forceinline__ ss_gen ss_generator(const double* g, double sc
 static __device__ __forceinline__ void ss_gen_apply(const ss_gen& a, const ss_cf x[SS_D][SS_D],
                                                     ss_cf out[SS_D][SS_D])
 {
+    // term by term over all entries (each entry keeps its summation order),
+    // so consecutive operations update different entries
 #pragma unroll
-    for (int c = 0; c < SS_D; ++c) {
+    for (int c = 0; c < SS_D; ++c)
 #pragma unroll
-        for (int r = 0; r < SS_D; ++r) {
-            ss_cf acc = ss_fimul(a.d[r], x[r][c]);
-            if (r > 0) acc = ss_fmac(acc, a.l, x[r - 1][c]);
-            if (r + 1 < SS_D) acc = ss_fmac(acc, a.u, x[r + 1][c]);
-            out[r][c] = acc;
-        }
-    }
+        for (int r = 0; r < SS_D; ++r) out[r][c] = ss_fimul(a.d[r], x[r][c]);
+#pragma unroll
+    for (int c = 0; c < SS_D; ++c)
+#pragma unroll
+        for (int r = 1; r < SS_D; ++r) out[r][c] = ss_fmac(out[r][c], a.l, x[r - 1][c]);
+#pragma unroll
+    for (int c = 0; c < SS_D; ++c)
+#pragma unroll
+        for (int r = 0; r + 1 < SS_D; ++r) out[r][c] = ss_fmac(out[r][c], a.u, x[r + 1][c]);
 }
 
 // Dense out = x y (float32)
 static __device__ __forceinline__ void ss_fmatmul(const ss_cf x[SS_D][SS_D], const ss_cf y[SS_D][SS_D],
                                                   ss_cf out[SS_D][SS_D])
 {
+    // k outermost: consecutive operations update different entries
 #pragma unroll
     for (int r = 0; r < SS_D; ++r)
 #pragma unroll
-        for (int c = 0; c < SS_D; ++c) {
-            ss_cf acc = ss_f(x[r][0].x * y[0][c].x - x[r][0].y * y[0][c].y,
+        for (int c = 0; c < SS_D; ++c)
+            out[r][c] = ss_f(x[r][0].x * y[0][c].x - x[r][0].y * y[0][c].y,
                              x[r][0].x * y[0][c].y + x[r][0].y * y[0][c].x);
 #pragma unroll
-            for (int k = 1; k < SS_D; ++k) acc = ss_fmac(acc, x[r][k], y[k][c]);
-            out[r][c] = acc;
-        }
+    for (int k = 1; k < SS_D; ++k)
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) out[r][c] = ss_fmac(out[r][c], x[r][k], y[k][c]);
 }
 
 // Dense float copy of A.

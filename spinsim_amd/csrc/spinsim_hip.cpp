@@ -848,6 +848,28 @@ __global__ void zero_counter(unsigned long long* p)
     if (threadIdx.x == 0) p[0] = 0ull;
 }
 
+// Sustained float64 FMA rate: 8 independent accumulator chains per lane,
+// `iters` rounds of 8 FMAs each; one result per thread keeps the work live.
+__global__ void __launch_bounds__(256) fma_probe_kernel(double* out, long long n, int iters)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = 1.0 + 1e-3 * (double)((i + k) & 1023);
+    const double b = 0.9999999999, c = 1e-12;
+#pragma unroll 1
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a[k] = fma(a[k], b, c);
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += a[k];
+    if (i < n) out[i] = sum;
+}
+
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct SingleScanLayout {
@@ -1450,6 +1472,18 @@ int ss_spin(const double* state, double* spin, int64_t n, int dim, ss_stream str
         hipLaunchKernelGGL(spin_kernel<2>, dim3(unsigned(blocks)), dim3(256), 0, s, state, spin, (long long)n);
     else
         hipLaunchKernelGGL(spin_kernel<3>, dim3(unsigned(blocks)), dim3(256), 0, s, state, spin, (long long)n);
+    SS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ss_fma_probe(double* out, int64_t threads, int iters, ss_stream stream)
+{
+    if (threads <= 0 || iters <= 0) return fail("ss_fma_probe: threads and iters must be positive");
+    if (!out) return fail("ss_fma_probe: null buffer");
+    const long long blocks = (threads + 255) / 256;
+    if (blocks > 2147483647LL) return fail("ss_fma_probe: too many threads");
+    hipLaunchKernelGGL(fma_probe_kernel, dim3(unsigned(blocks)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       out, (long long)threads, iters);
     SS_HIP(hipGetLastError());
     return 0;
 }

@@ -49,8 +49,13 @@ def test_library_rejects_bad_arguments():
     assert lib.ss_spin(None, None, -1, 2, None) != 0
     assert lib.ss_state_scan_workspace(0, 2) == 0
     assert lib.ss_state_scan_workspace(100, 3) > 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 3, 10, 2, None, 0, None) != 0
+    assert b"null buffer" in lib.ss_last_error()
+    assert lib.ss_time_evolution(None, None, 0.0, 1e-7, 1e-6, 10, None, None, None, 65, None) != 0
+    assert lib.ss_fma_probe(None, 0, 1, None) != 0
     # zero-sized work is a no-op success, even with null buffers
     assert lib.ss_spin(None, None, 0, 2, None) == 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, None, 0, None) == 0
 
 
 @pytest.mark.parametrize("dim,expo", [(2, 0), (2, 1), (3, 1)])
