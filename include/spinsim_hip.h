@@ -77,9 +77,12 @@ enum { /* arithmetic of the time-evolution kernel (no reference counterpart) */
                                   float32; samples, first-order terms, products
                                   across lanes, outputs and scans float64 */
 };
-enum { /* ExponentiationMethod, :95-129 (LIE_TROTTER_8 not provided) */
+enum { /* ExponentiationMethod, :95-129 */
     SS_EXPO_ANALYTIC = 0,
-    SS_EXPO_LIE_TROTTER = 1
+    SS_EXPO_LIE_TROTTER = 1,
+    SS_EXPO_LIE_TROTTER_8 = 2 /* spin one, float64: the exponential of the eight
+                                 generators documented at :1422-1484 (field entries
+                                 x, y, z, q, u1, u2, v1, v2) */
 };
 
 /* Message of the last failure on this thread ("" if none). */

@@ -27,7 +27,7 @@ _LOCK = threading.Lock()
 _CACHE: dict = {}
 
 METHOD_INDEX = {"magnus_cf4": 0, "euler": 1, "heun": 2}
-EXPO_INDEX = {"analytic": 0, "lie_trotter": 1}
+EXPO_INDEX = {"analytic": 0, "lie_trotter": 1, "lie_trotter_8": 2}
 
 PRELUDE = r"""
 #include <math.h>
@@ -92,7 +92,7 @@ class COracle:
         self.dim = dim
         self.sampler_names = tuple(sampler_names)
         defines = {
-            "SS_DIM": dim, "SS_LIE": 3 if dim == 2 else 4,
+            "SS_DIM": dim, "SS_LIE": 3 if dim == 2 else (8 if expo == "lie_trotter_8" else 4),
             "SS_METHOD": METHOD_INDEX[method], "SS_EXPO": EXPO_INDEX[expo],
             "SS_ROT": int(bool(rotating)), "SS_HYPER": _hyper(number_of_squares),
             "SS_CTX": int(bool(self.sampler_names)),

@@ -11,8 +11,9 @@
  * sequential state product of get_state (:1103-1111).  It is compiled once per
  * field callback by oracle/c_oracle.py, which prepends a prelude and the
  * callback's source (the same text the GPU path compiles) and sets:
- *   SS_DIM 2|3, SS_LIE 3|4, SS_METHOD 0 (magnus_cf4) | 1 (euler) | 2 (heun),
- *   SS_EXPO 0 (analytic) | 1 (lie_trotter), SS_ROT 0|1, SS_HYPER (hypercubes),
+ *   SS_DIM 2|3, SS_LIE 3|4|8, SS_METHOD 0 (magnus_cf4) | 1 (euler) | 2 (heun),
+ *   SS_EXPO 0 (analytic) | 1 (lie_trotter) | 2 (eight-generator lie_trotter),
+ *   SS_ROT 0|1, SS_HYPER (hypercubes),
  *   SS_CTX 0|1 (callback takes the sampler context argument).
  * Built with -std=c11 (no FP contraction) and without -ffast-math.
  */
@@ -133,6 +134,21 @@ static inline void square_m1(const cplx *op, cplx *out)
 
 #define TROTTER_PRECISION ((double)(1ull << (2 * SS_HYPER)))
 
+#if SS_EXPO == 2
+/* exp(-i angle (cos p Jx + sin p Jy)) - I for spin one, angle = |(gx, gy)| scale,
+   exp(ip) = (gx + i gy)/|.|; sign2 = -1 conjugates by diag(1, 1, -1) (V1/V2). */
+static inline void lt8_rotation_m1(double gx, double gy, double scale, double sign2, cplx *r)
+{
+    double n = sqrt(gx * gx + gy * gy);
+    cplx ep = (n > 0) ? (gx + I * gy) / n : 1;
+    double angle = n * scale, sh = sin(angle / 2);
+    cplx sa = -I * sin(angle) / SQRT2, epc = conj(ep);
+    r[0] = -(sh * sh); r[1] = sa * epc; r[2] = -(sh * sh) * epc * epc * sign2;
+    r[3] = sa * ep; r[4] = -2 * (sh * sh); r[5] = sa * epc * sign2;
+    r[6] = -(sh * sh) * ep * ep * sign2; r[7] = sa * ep * sign2; r[8] = -(sh * sh);
+}
+#endif
+
 /* ---- exponentials, :1617-1673, :1767-1807 ------------------------------ */
 static inline void expo(const double *g, cplx *res)
 {
@@ -163,6 +179,32 @@ static inline void expo(const double *g, cplx *res)
     res[3] = cos_exp_m1(a / 2, z);
     for (int h = 0; h < SS_HYPER; ++h) { square_m1(res, tmp); square_m1(tmp, res); }
     res[0] += 1; res[3] += 1;
+#elif SS_EXPO == 2
+    /* eight generators, symmetric product D(1/2) V(1/2) U(1/2) J U(1/2) V(1/2) D(1/2)
+       of exact exponentials, as residuals, then the residual squarings */
+    const double s = 1.0 / TROTTER_PRECISION;
+    cplx fj[9], fu[9], fv[9], fd[9], tmp[9], fr[9];
+    lt8_rotation_m1(g[0], g[1], s, 1.0, fj);
+    lt8_rotation_m1(g[6], g[7], s / 2, -1.0, fv);
+    double au = sqrt(g[4] * g[4] + g[5] * g[5]);
+    cplx epu = (au > 0) ? (g[4] + I * g[5]) / au : 1;
+    double th = au * s / 2, shu = sin(th / 2);
+    for (int k = 0; k < 9; ++k) { fu[k] = 0; fd[k] = 0; }
+    fu[0] = fu[8] = -2 * (shu * shu);
+    fu[2] = -I * sin(th) * conj(epu);
+    fu[6] = -I * sin(th) * epu;
+    const double h = s / 2;
+    fd[0] = expm1i(-h * (g[2] + g[3] / 3));
+    fd[4] = expm1i(h * 2 * g[3] / 3);
+    fd[8] = expm1i(-h * (-g[2] + g[3] / 3));
+    const cplx *order[6] = {fv, fu, fj, fu, fv, fd};
+    memcpy(res, fd, sizeof(fd));
+    for (int f = 0; f < 6; ++f) {
+        matmul(order[f], res, fr);
+        for (int k = 0; k < 9; ++k) res[k] = order[f][k] + res[k] + fr[k];
+    }
+    for (int hh = 0; hh < SS_HYPER; ++hh) { square_m1(res, tmp); square_m1(tmp, res); }
+    res[0] += 1; res[4] += 1; res[8] += 1;
 #else
     double a = sqrt(g[0] * g[0] + g[1] * g[1]);
     double p = (a > 0) ? atan2(g[1], g[0]) : 0;
@@ -200,6 +242,13 @@ static inline void transform_frame(double *f, double rot, cplx w)
     cplx x = (f[0] + I * f[1]) / w;
     f[0] = creal(x); f[1] = cimag(x);
     f[2] = f[2] - rot;
+#if SS_LIE == 8
+    /* :586-596: U1/U2 turn at twice the frame rate, V1/V2 at the frame rate */
+    x = (f[4] + I * f[5]) * conj(w) * conj(w);
+    f[4] = creal(x); f[5] = cimag(x);
+    x = (f[6] + I * f[7]) * conj(w);
+    f[6] = creal(x); f[7] = cimag(x);
+#endif
 #endif
 #else
     (void)f; (void)rot; (void)w;

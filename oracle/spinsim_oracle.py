@@ -209,6 +209,73 @@ def matrix_exponential_lie_trotter_one(g, number_of_squares=24):
     return res
 
 
+def _expm1i(b):
+    """exp(ib) - 1 without cancellation."""
+    return -2 * np.sin(b / 2) ** 2 + 1j * np.sin(b)
+
+
+def _rotation_one_m1(angle, ep, sign2):
+    """exp(-i angle (cos p Jx + sin p Jy)) - I for spin one, ep = exp(ip);
+    with sign2 = -1 the same conjugated by diag(1, 1, -1) (the V1/V2 pair)."""
+    sh = np.sin(angle / 2)
+    sa = -1j * np.sin(angle) / SQRT2
+    epc = np.conj(ep)
+    d = -(sh ** 2) + 0j
+    return [[d, sa * epc, -(sh ** 2) * epc ** 2 * sign2],
+            [sa * ep, -2 * sh ** 2 + 0j, sa * epc * sign2],
+            [-(sh ** 2) * ep ** 2 * sign2, sa * ep * sign2, d]]
+
+
+def _residual_product(f, r):
+    """(I + f)(I + r) - I = f + r + f r."""
+    fr = matrix_multiply(f, r)
+    return [[f[i][k] + r[i][k] + fr[i][k] for k in range(3)] for i in range(3)]
+
+
+def matrix_exponential_lie_trotter_eight(g, number_of_squares=24):
+    """Spin-one exponential of -i(x Jx + y Jy + z Jz + q Q + u1 U1 + u2 U2 +
+    v1 V1 + v2 V2) with the generators of the reference's documentation,
+    :1422-1484.  The reference's own matrix_exponential_lie_trotter_8
+    (:1810-1925) is not a usable specification (it reuses a stale variable and
+    fills entries that must be zero), so this is the documented formula done
+    as a symmetric Lie-Trotter product at scale 4**-hypercubes,
+        T = D(1/2) V(1/2) U(1/2) J(1) U(1/2) V(1/2) D(1/2),
+    each factor an exact exponential kept as its residual T - I, followed by
+    the same residual squarings as the other Lie-Trotter exponentials."""
+    hyper, precision = _hypercubes(number_of_squares)
+    gx, gy, gz, gq, u1, u2, v1, v2 = (np.asarray(v, np.float64) for v in g[:8])
+    s = 1.0 / precision
+
+    def phase(a, b):
+        n = np.sqrt(a * a + b * b)
+        safe = np.where(n > 0, n, 1.0)
+        return n, np.where(n > 0, (a + 1j * b) / safe, 1.0 + 0j)
+
+    a_j, ep_j = phase(gx, gy)
+    a_u, ep_u = phase(u1, u2)
+    a_v, ep_v = phase(v1, v2)
+    zero = np.zeros_like(gx) + 0j
+    f_j = _rotation_one_m1(a_j * s, ep_j, 1.0)
+    f_v = _rotation_one_m1(a_v * s / 2, ep_v, -1.0)
+    th = a_u * s / 2
+    cu = -2 * np.sin(th / 2) ** 2 + 0j
+    su = -1j * np.sin(th)
+    f_u = [[cu, zero, su * np.conj(ep_u)], [zero, zero, zero], [su * ep_u, zero, cu]]
+    h = s / 2
+    f_d = [[_expm1i(-h * (gz + gq / 3)), zero, zero],
+           [zero, _expm1i(h * 2 * gq / 3), zero],
+           [zero, zero, _expm1i(-h * (-gz + gq / 3))]]
+    res = f_d
+    for f in (f_v, f_u, f_j, f_u, f_v, f_d):
+        res = _residual_product(f, res)
+    res = [[np.asarray(v, np.complex128) for v in row] for row in res]
+    for _ in range(hyper):
+        res = matrix_square_m1(matrix_square_m1(res))
+    for k in range(3):
+        res[k][k] = res[k][k] + 1
+    return res
+
+
 # --------------------------------------------------------------------------
 # Rotating frame, :582-618 (frame entry) and :838-852 (frame exit)
 # --------------------------------------------------------------------------
@@ -227,6 +294,14 @@ def transform_frame(field, rotating_wave, winding, dim, rotating):
         field[0] = x.real
         field[1] = x.imag
         field[2] = field[2] - rotating_wave
+        if len(field) > 4:
+            # transform_frame_spin_one_rotating_8, :586-596: U rotates twice as fast
+            x = (field[4] + 1j * field[5]) * np.conj(winding) * np.conj(winding)
+            field[4] = x.real
+            field[5] = x.imag
+            x = (field[6] + 1j * field[7]) * np.conj(winding)
+            field[6] = x.real
+            field[7] = x.imag
 
 
 # --------------------------------------------------------------------------
@@ -238,10 +313,10 @@ def resolve_methods(spin, exponentiation=None, integration=MAGNUS_CF4):
         exponentiation = LIE_TROTTER if spin == ONE else ANALYTIC
     if exponentiation == ANALYTIC and spin != HALF:
         exponentiation = LIE_TROTTER
-    if exponentiation == LIE_TROTTER_8:
-        raise NotImplementedError("LIE_TROTTER_8 is not part of the oracle")
+    if exponentiation == LIE_TROTTER_8 and spin != ONE:
+        raise ValueError("LIE_TROTTER_8 is only defined for spin one")  # :1429-1432
     dim = 2 if spin == HALF else 3
-    lie = 3 if spin == HALF else 4
+    lie = 3 if spin == HALF else (8 if exponentiation == LIE_TROTTER_8 else 4)
     return dim, lie, exponentiation, integration
 
 
@@ -250,6 +325,8 @@ def _exponential(g, dim, exponentiation, number_of_squares):
         return matrix_exponential_analytic(g)
     if dim == 2:
         return matrix_exponential_lie_trotter_half(g, number_of_squares)
+    if exponentiation == LIE_TROTTER_8:
+        return matrix_exponential_lie_trotter_eight(g, number_of_squares)
     return matrix_exponential_lie_trotter_one(g, number_of_squares)
 
 

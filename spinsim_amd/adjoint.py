@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _native
 from .batch import BatchPlan
-from .enums import IntegrationMethod
+from .enums import ExponentiationMethod, IntegrationMethod
 from .interpolation import InterpolationSampler
 from .simulator import FieldSource, Simulator
 
@@ -82,6 +82,9 @@ class ControlledSimulator:
             raise ValueError("1 to 4 controls")
         if spin_quantum_number.dimension == 2 and any(c.axis == 3 for c in self.controls):
             raise ValueError("the quadratic shift (axis q) exists for spin one only")
+        if exponentiation_method == ExponentiationMethod.LIE_TROTTER_8:
+            raise ValueError("the adjoint differentiates the x, y, z and q generators; "
+                             "ExponentiationMethod.LIE_TROTTER_8 is not supported here")
         static = FieldSource(static_field, samplers) if static_field is not None else None
         src = _combined_source(static, self.controls)
         table_samplers = (list(static.samplers) if static is not None else []) + \

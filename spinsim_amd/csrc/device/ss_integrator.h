@@ -1,9 +1,9 @@
 // spinsim_amd time-evolution kernels (compiled by hipRTC after the field
 // callback, specialised by these macros):
 //   SS_DIM 2|3          spin-half / spin-one
-//   SS_LIE 3|4          field entries used (x, y, z[, q])
+//   SS_LIE 3|4|8        field entries used (x, y, z[, q[, u1, u2, v1, v2]])
 //   SS_METHOD 0|1|2     magnus_cf4 / midpoint sample (euler) / half step (heun)
-//   SS_EXPO 0|1         analytic (spin-half) / Lie-Trotter
+//   SS_EXPO 0|1|2       analytic (spin-half) / Lie-Trotter / eight-generator Lie-Trotter
 //   SS_ROT 0|1          rotating frame
 //   SS_HYPER h          Lie-Trotter hypercubes (2h residual squarings)
 //   SS_CTX 0|1          callback takes the sampler context argument
@@ -122,6 +122,119 @@ static __device__ __forceinline__ void ss_expo(const double g[SS_FIELD_LEN], ss_
     e[0][0].x += 1.0;
     e[1][1].x += 1.0;
 }
+#elif SS_EXPO == 2
+// Eight generators (spin one): exp(-i(x Jx + y Jy + z Jz + q Q + u1 U1 + u2 U2
+// + v1 V1 + v2 V2)) with the generators the reference documents (:1422-1484).
+// The reference's own matrix_exponential_lie_trotter_8 (:1810-1925) reuses a
+// stale variable and fills entries that must be zero, so this computes the
+// documented exponential as a symmetric Lie-Trotter product at scale
+// 4^-hypercubes, D(1/2) V(1/2) U(1/2) J U(1/2) V(1/2) D(1/2), of exact
+// factor exponentials kept as residuals (T - I), then squares like the other
+// Lie-Trotter exponentials.
+
+// exp(-i angle (cos p Jx + sin p Jy)) - I, angle = |(gx, gy)| scale,
+// exp(ip) = (gx + i gy)/|.|; sign2 = -1 conjugates it by diag(1, 1, -1),
+// which turns (Jx, Jy) into (V1, V2).
+static __device__ __forceinline__ void ss_lt8_rotation_m1(double gx, double gy, double scale, double sign2,
+                                                          ss_cd r[3][3])
+{
+    const double n = sqrt(gx * gx + gy * gy);
+    ss_cd ep = ss_c(1.0, 0.0);
+    if (n > 0) {
+        const double in = 1.0 / n;
+        ep = ss_c(gx * in, gy * in);
+    }
+    double sh, ch;
+    ss_sincos(0.5 * n * scale, &sh, &ch);
+    const ss_cd sa = ss_c(0.0, -2.0 * sh * ch * SS_INV_SQRT2);  // -i sin(angle)/sqrt2
+    const double d = -sh * sh;                                   // cos^2(angle/2) - 1
+    const ss_cd ep2 = ss_mul(ep, ep);
+    const ss_cd sep = ss_mul(sa, ep);
+    const ss_cd sem = ss_mul(sa, ss_conj(ep));
+    r[0][0] = ss_c(d, 0.0);
+    r[0][1] = sem;
+    r[0][2] = ss_scale(ss_conj(ep2), d * sign2);
+    r[1][0] = sep;
+    r[1][1] = ss_c(2.0 * d, 0.0);
+    r[1][2] = ss_scale(sem, sign2);
+    r[2][0] = ss_scale(ep2, d * sign2);
+    r[2][1] = ss_scale(sep, sign2);
+    r[2][2] = ss_c(d, 0.0);
+}
+
+// exp(-i th (cos p U1 + sin p U2)) - I, th = |(u1, u2)| scale.
+static __device__ __forceinline__ void ss_lt8_u_m1(double u1, double u2, double scale, ss_cd r[3][3])
+{
+    const double n = sqrt(u1 * u1 + u2 * u2);
+    ss_cd ep = ss_c(1.0, 0.0);
+    if (n > 0) {
+        const double in = 1.0 / n;
+        ep = ss_c(u1 * in, u2 * in);
+    }
+    double sh, ch;
+    ss_sincos(0.5 * n * scale, &sh, &ch);
+    const ss_cd si = ss_c(0.0, -2.0 * sh * ch);                  // -i sin(th)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[i][k] = ss_c(0.0, 0.0);
+    r[0][0] = ss_c(-2.0 * sh * sh, 0.0);
+    r[2][2] = r[0][0];
+    r[0][2] = ss_mul(si, ss_conj(ep));
+    r[2][0] = ss_mul(si, ep);
+}
+
+// exp(-i h (z Jz + q Q)) - I, diagonal.
+static __device__ __forceinline__ void ss_lt8_d_m1(double z, double q, double h, ss_cd r[3][3])
+{
+    const double b[3] = {-h * (z + q / 3.0), h * 2.0 * q / 3.0, -h * (-z + q / 3.0)};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[i][k] = ss_c(0.0, 0.0);
+        double sb, cb;
+        ss_sincos(0.5 * b[i], &sb, &cb);
+        r[i][i] = ss_c(-2.0 * sb * sb, 2.0 * sb * cb);        // exp(ib) - 1
+    }
+}
+
+// r <- f + r + f r, i.e. (I + f)(I + r) - I
+static __device__ __forceinline__ void ss_lt8_fold(const ss_cd f[3][3], ss_cd r[3][3])
+{
+    ss_cd fr[3][3];
+    ss_matmul<3>(f, r, fr);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[i][k] = ss_add(ss_add(f[i][k], r[i][k]), fr[i][k]);
+}
+
+static __device__ __forceinline__ void ss_expo(const double g[SS_FIELD_LEN], ss_cd e[3][3])
+{
+    const double s = SS_INV_P;
+    ss_cd f[3][3];
+    ss_lt8_d_m1(g[2], g[3], 0.5 * s, e);
+    ss_lt8_rotation_m1(g[6], g[7], 0.5 * s, -1.0, f);
+    ss_lt8_fold(f, e);
+    ss_lt8_u_m1(g[4], g[5], 0.5 * s, f);
+    ss_lt8_fold(f, e);
+    ss_lt8_rotation_m1(g[0], g[1], s, 1.0, f);
+    ss_lt8_fold(f, e);
+    ss_lt8_u_m1(g[4], g[5], 0.5 * s, f);
+    ss_lt8_fold(f, e);
+    ss_lt8_rotation_m1(g[6], g[7], 0.5 * s, -1.0, f);
+    ss_lt8_fold(f, e);
+    ss_lt8_d_m1(g[2], g[3], 0.5 * s, f);
+    ss_lt8_fold(f, e);
+#pragma unroll 1
+    for (int h = 0; h < SS_HYPER; ++h) {
+        ss_square_m1(e);
+        ss_square_m1(e);
+    }
+    e[0][0].x += 1.0;
+    e[1][1].x += 1.0;
+    e[2][2].x += 1.0;
+}
 #else
 // matrix_exponential_lie_trotter (spin one), spinsim/__init__.py:1767-1807.
 // The reference forms each entry from atan2 and complex exponentials of
@@ -208,6 +321,17 @@ static __device__ __forceinline__ void ss_frame(double f[SS_FIELD_LEN], double r
 #else
     const ss_cd x = ss_mul(ss_c(f[0], f[1]), ss_conj(w));
     f[2] = f[2] - rot;
+#if SS_LIE == 8
+    // transform_frame_spin_one_rotating_8, :586-596: U1/U2 turn at twice the
+    // frame rate, V1/V2 at the frame rate
+    const ss_cd wc = ss_conj(w);
+    const ss_cd xu = ss_mul(ss_c(f[4], f[5]), ss_mul(wc, wc));
+    const ss_cd xv = ss_mul(ss_c(f[6], f[7]), wc);
+    f[4] = xu.x;
+    f[5] = xu.y;
+    f[6] = xv.x;
+    f[7] = xv.y;
+#endif
 #endif
     f[0] = x.x;
     f[1] = x.y;
@@ -443,7 +567,7 @@ static __device__ __forceinline__ void ss_append_mixed(const double g[SS_FIELD_L
 static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], const ss_cf b[SS_D][SS_D],
                                                      ss_cd u[SS_D][SS_D])
 {
-    double gg[4] = {gsum[0], gsum[1], gsum[2], SS_LIE > 3 ? gsum[SS_LIE - 1] : 0.0};
+    double gg[4] = {gsum[0], gsum[1], gsum[2], SS_LIE > 3 ? gsum[3] : 0.0};
 #if SS_DIM == 2
     const double hx = 0.5 * gg[0], hy = 0.5 * gg[1], hz = 0.5 * gg[2];
     const double d[2] = {-hz, hz};
@@ -997,7 +1121,7 @@ static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], dou
     out[0] = drot[0];
     out[1] = drot[1];
     out[2] = drot[2];
-    out[3] = SS_LIE > 3 ? drot[SS_LIE - 1] : 0.0;
+    out[3] = SS_LIE > 3 ? drot[3] : 0.0;
 #if SS_ROT
     const ss_cd w = ss_winding(rot, t_rel);
 #if SS_DIM == 2
@@ -1043,7 +1167,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
     long long base[SS_MAX_CONTROLS];
     for (int c = 0; c < a.n_controls; ++c) {
         const int ax = a.controls[c].axis;
-        mask |= (ax <= 1 && SS_ROT) ? 3u : (1u << (ax == 3 ? SS_LIE - 1 : ax));
+        mask |= (ax <= 1 && SS_ROT) ? 3u : (1u << ax);
         double p;
         const long long i0 = ss_find_time_index(&ctx.tables[a.controls[c].table], tc, &p);
         base[c] = (i0 < 0 ? 0 : i0) - 1;

@@ -922,7 +922,8 @@ std::string build_source(const char* field_src, int dim, int method, int expo, i
     snprintf(head, sizeof(head),
              "#define SS_DIM %d\n#define SS_LIE %d\n#define SS_METHOD %d\n#define SS_EXPO %d\n"
              "#define SS_ROT %d\n#define SS_HYPER %d\n#define SS_CTX %d\n#define SS_PREC %d\n",
-             dim, dim == 2 ? 3 : 4, method, expo, rotating ? 1 : 0, hyper, field_has_context ? 1 : 0,
+             dim, dim == 2 ? 3 : (expo == SS_EXPO_LIE_TROTTER_8 ? 8 : 4), method, expo, rotating ? 1 : 0, hyper,
+             field_has_context ? 1 : 0,
              precision);
     s += head;
     s += ss_src_args;
@@ -968,8 +969,13 @@ int ss_compile_code(const char* field_src, int dim, int method, int expo, int ro
     if (!field_src || !code || !code_size) return fail("ss_compile_code: null argument");
     if (dim != 2 && dim != 3) return fail("ss_compile_code: dim must be 2 or 3, got %d", dim);
     if (method < 0 || method > 2) return fail("ss_compile_code: bad method %d", method);
-    if (expo != 0 && expo != 1) return fail("ss_compile_code: bad exponentiation method %d", expo);
+    if (expo < 0 || expo > 2) return fail("ss_compile_code: bad exponentiation method %d", expo);
     if (expo == 0 && dim != 2) return fail("ss_compile_code: analytic exponentiation is spin-half only");
+    if (expo == SS_EXPO_LIE_TROTTER_8 && dim != 3)
+        return fail("ss_compile_code: eight-generator Lie-Trotter exponentiation is spin-one only");
+    if (expo == SS_EXPO_LIE_TROTTER_8 && precision == SS_PRECISION_FP32_BULK)
+        return fail("ss_compile_code: the float32-bulk mode covers x, y, z and q only, not the "
+                    "eight-generator exponentiation");
     if (number_of_squares > 60) return fail("ss_compile_code: number_of_squares %d too large", number_of_squares);
     if (precision != SS_PRECISION_FP64 && precision != SS_PRECISION_FP32_BULK)
         return fail("ss_compile_code: bad precision %d", precision);
@@ -1390,6 +1396,8 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
                      int n_controls, double* grad, int64_t grad_stride, int32_t* status, ss_stream stream)
 {
     if (!m) return fail("ss_batch_adjoint: null module");
+    if (m->expo == SS_EXPO_LIE_TROTTER_8)
+        return fail("ss_batch_adjoint: the adjoint covers x, y, z and q, not the eight-generator exponentiation");
     if (n_sims < 0 || total < 0) return fail("ss_batch_adjoint: negative size");
     if (n_controls < 1 || n_controls > SS_MAX_CONTROLS)
         return fail("ss_batch_adjoint: 1 to %d controls, got %d", SS_MAX_CONTROLS, n_controls);

@@ -161,9 +161,10 @@ def resolve_exponentiation(spin_quantum_number, exponentiation_method, warn=True
             print(f"{_RED}spinsim warning!!!\n_attempting to use an analytic exponentiation "
                   f"method outside of spin-half. Switching to a Lie Trotter method.{_RESET}")
         exponentiation_method = ExponentiationMethod.LIE_TROTTER
-    if exponentiation_method == ExponentiationMethod.LIE_TROTTER_8:
-        raise NotImplementedError(
-            "ExponentiationMethod.LIE_TROTTER_8 is not provided by spinsim_amd; use LIE_TROTTER")
+    if (exponentiation_method == ExponentiationMethod.LIE_TROTTER_8
+            and spin_quantum_number != SpinQuantumNumber.ONE):
+        # "Only available for use with spin-one systems", :1429-1432
+        raise ValueError("ExponentiationMethod.LIE_TROTTER_8 is only available for spin-one systems")
     return exponentiation_method
 
 
@@ -226,6 +227,9 @@ class Simulator:
         self.precision = precision
         self.exponentiation_method = resolve_exponentiation(spin_quantum_number,
                                                             exponentiation_method)
+        if precision == "fp32_bulk" and self.exponentiation_method == ExponentiationMethod.LIE_TROTTER_8:
+            raise ValueError("precision='fp32_bulk' covers the x, y, z and q generators only; "
+                             "use precision='fp64' with ExponentiationMethod.LIE_TROTTER_8")
         self.dimension = spin_quantum_number.dimension
         self._modules = {}
         self._tables = {}

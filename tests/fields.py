@@ -95,6 +95,24 @@ __device__ void get_field(double t, const double* s, double* f)
 """, _tilted_one, "one")
 
 
+def _tilted_eight(t, s, f):
+    _tilted_one(t, s, f)
+    f[4] = TAU * 8e3 * np.cos(TAU * 1.5e4 * t)
+    f[5] = TAU * 3e3
+    f[6] = TAU * 6e3 * np.sin(TAU * 2.5e4 * t) - TAU * 2e3
+    f[7] = TAU * 4e3 * np.cos(TAU * 5e3 * t)
+
+
+# All eight su(3) directions of the eight-generator Lie-Trotter exponential.
+TILTED_EIGHT = Field("tilted_eight", """
+__device__ void get_field(double t, const double* s, double* f)
+{ f[0] = SS_TAU * 3e4 * cos(SS_TAU * 2e4 * t) + SS_TAU * 5e3; f[1] = SS_TAU * 1e4 * sin(SS_TAU * 3e4 * t);
+  f[2] = SS_TAU * 2e4 + SS_TAU * 5e3 * sin(SS_TAU * 1e3 * t); f[3] = SS_TAU * 4e3 * cos(SS_TAU * 7e3 * t);
+  f[4] = SS_TAU * 8e3 * cos(SS_TAU * 1.5e4 * t); f[5] = SS_TAU * 3e3;
+  f[6] = SS_TAU * 6e3 * sin(SS_TAU * 2.5e4 * t) - SS_TAU * 2e3; f[7] = SS_TAU * 4e3 * cos(SS_TAU * 5e3 * t); }
+""", _tilted_eight, "one")
+
+
 def _tilted_half(t, s, f):
     f[0] = TAU * 3e4 * np.cos(TAU * 2e4 * t) + TAU * 5e3
     f[1] = TAU * 1e4 * np.sin(TAU * 3e4 * t)

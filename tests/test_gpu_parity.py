@@ -32,7 +32,8 @@ def run_product(field, t0, t1, dt, dt_out, psi0, method="magnus_cf4", expo=None,
                 lanes=None, squares=24, sweep=None):
     ss = _ss()
     e = None if expo is None else {"analytic": ss.ExponentiationMethod.ANALYTIC,
-                                   "lie_trotter": ss.ExponentiationMethod.LIE_TROTTER}[expo]
+                                   "lie_trotter": ss.ExponentiationMethod.LIE_TROTTER,
+                                   "lie_trotter_8": ss.ExponentiationMethod.LIE_TROTTER_8}[expo]
     sim = ss.Simulator(field.src, _spin_enum(field.spin), exponentiation_method=e,
                        use_rotating_frame=rotating,
                        integration_method=getattr(ss.IntegrationMethod, METHODS[method]),
@@ -112,6 +113,46 @@ def test_methods_frames(field, method, expo, rotating):
     res = run_product(*args, method=method, expo=expo, rotating=rotating)
     ref = run_oracle(*args, method=method, expo=expo, rotating=rotating)
     assert_close(res, ref)
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("rotating", [True, False])
+def test_eight_generator_lie_trotter(method, rotating):
+    """ExponentiationMethod.LIE_TROTTER_8: all eight su(3) directions driven,
+    both frames (U1/U2 and V1/V2 rotate at 2x and 1x the frame rate)."""
+    print(f"case: lie_trotter_8 {method} rotating={rotating}", flush=True)
+    psi0 = O.EIGENSTATES["one"]["plus_x"]
+    args = (F.TILTED_EIGHT, 1e-5, 1e-5 + 4e-4, 1e-7, 1e-6, psi0)
+    res = run_product(*args, method=method, expo="lie_trotter_8", rotating=rotating)
+    ref = run_oracle(*args, method=method, expo="lie_trotter_8", rotating=rotating)
+    assert_close(res, ref)
+
+
+def test_eight_generator_python_callback():
+    """A numba-style Python field writing all eight entries, translated to HIP-C."""
+    ss = _ss()
+    print("case: lie_trotter_8 python callback", flush=True)
+
+    def get_field(time_sample, sweep_parameters, field_sample):
+        field_sample[0] = math.tau * 3e4 * math.cos(math.tau * 2e4 * time_sample)
+        field_sample[1] = 0
+        field_sample[2] = math.tau * 2e4
+        field_sample[3] = math.tau * 4e3
+        field_sample[4] = math.tau * 8e3 * math.cos(math.tau * 1.5e4 * time_sample)
+        field_sample[5] = math.tau * 3e3
+        field_sample[6] = math.tau * 6e3 * math.sin(math.tau * 2.5e4 * time_sample)
+        field_sample[7] = -math.tau * 2e3
+
+    sim = ss.Simulator(get_field, ss.SpinQuantumNumber.ONE,
+                       exponentiation_method=ss.ExponentiationMethod.LIE_TROTTER_8)
+    psi0 = O.EIGENSTATES["one"]["plus_z"]
+    res = sim.evaluate(0, 2e-4, 1e-7, 1e-6, psi0)
+    src, _, _ = ss.transpile(get_field)
+    co = COracle(src, 3, "magnus_cf4", "lie_trotter_8")
+    dt, n, nf, _ = O.timing(0, 2e-4, 1e-7, 1e-6)
+    _, u = co.time_evolution([0.0], 0.0, dt, 1e-6, nf, n)
+    assert np.abs(res.time_evolution - u).max() <= TOL
+    assert np.abs(res.state - co.get_state(psi0, u)).max() <= TOL
 
 
 @pytest.mark.parametrize("lanes", [1, 2, 3, 4, 5, 7, 8, 10, 16, 64])

@@ -58,11 +58,12 @@ def test_library_rejects_bad_arguments():
     assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, None, 0, None) == 0
 
 
-@pytest.mark.parametrize("dim,expo", [(2, 0), (2, 1), (3, 1)])
+@pytest.mark.parametrize("dim,expo", [(2, 0), (2, 1), (3, 1), (3, 2)])
 @pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("rotating", [True, False])
 def test_hiprtc_compiles_every_variant(dim, expo, method, rotating):
-    src = F.RABI_SWEEP_ONE.src if dim == 3 else F.RABI_SWEEP_HALF.src
+    src = (F.TILTED_EIGHT.src if expo == 2 else
+           F.RABI_SWEEP_ONE.src if dim == 3 else F.RABI_SWEEP_HALF.src)
     code, _log = _native.compile_code(src, dim, method, expo, rotating, 24, False)
     assert code[:4] == b"\x7fELF"
 
@@ -74,6 +75,10 @@ def test_hiprtc_reports_compile_errors():
     assert "undefined_symbol" in str(err.value)
     with pytest.raises(_native.NativeError):
         _native.compile_code(F.LARMOR.src, 3, 0, 0, True, 24, False)  # analytic is spin-half only
+    with pytest.raises(_native.NativeError, match="spin-one only"):
+        _native.compile_code(F.LARMOR.src, 2, 0, 2, True, 24, False)
+    with pytest.raises(_native.NativeError, match="float32-bulk"):
+        _native.compile_code(F.TILTED_EIGHT.src, 3, 0, 2, True, 24, False, 1)
 
 
 def test_simulator_compile_error_message(capsys):
@@ -109,8 +114,9 @@ def test_exponentiation_defaults_and_fallback(capsys):
     assert resolve_exponentiation(ss.SpinQuantumNumber.ONE, None) == E.LIE_TROTTER
     assert resolve_exponentiation(ss.SpinQuantumNumber.ONE, E.ANALYTIC) == E.LIE_TROTTER
     assert "Switching to a Lie Trotter method" in capsys.readouterr().out
-    with pytest.raises(NotImplementedError):
-        resolve_exponentiation(ss.SpinQuantumNumber.ONE, E.LIE_TROTTER_8)
+    assert resolve_exponentiation(ss.SpinQuantumNumber.ONE, E.LIE_TROTTER_8) == E.LIE_TROTTER_8
+    with pytest.raises(ValueError, match="spin-one"):
+        resolve_exponentiation(ss.SpinQuantumNumber.HALF, E.LIE_TROTTER_8)
 
 
 def test_step_arithmetic_matches_oracle(capsys):

@@ -24,9 +24,16 @@ J1 = [np.array([[0, 1, 0], [1, 0, 1], [0, 1, 0]]) / SQ2,
       np.diag([1.0, 0, -1]), np.diag([1.0, -2, 1]) / 3]
 
 
+# The further su(3) directions of the eight-generator exponential (:1458-1484).
+J8 = J1 + [np.array([[0, 0, 1], [0, 0, 0], [1, 0, 0]]),
+           np.array([[0, 0, -1j], [0, 0, 0], [1j, 0, 0]]),
+           np.array([[0, 1, 0], [1, 0, -1], [0, -1, 0]]) / SQ2,
+           np.array([[0, -1j, 0], [1j, 0, 1j], [0, -1j, 0]]) / SQ2]
+
+
 def hamiltonian(f, spin):
-    ops = JH if spin == "half" else J1
-    return sum(f[k] * ops[k] for k in range(len(ops)))
+    ops = JH if spin == "half" else (J8 if len(f) > 4 else J1)
+    return sum(f[k] * ops[k] for k in range(min(len(ops), len(f))))
 
 
 def to_mat(entries):
@@ -50,6 +57,43 @@ def test_exponentials_match_expm(g):
     ref_1 = expm(-1j * hamiltonian(g, "one"))
     e = to_mat(O.matrix_exponential_lie_trotter_one(cols))[:, :, 0]
     assert np.abs(e - ref_1).max() < 1e-12
+
+
+G8_SAMPLES = [np.array(v, float) for v in (
+    [0.3, -0.2, 0.5, 0.1, 0.4, -0.3, 0.2, 0.6], [1e-9, 2e-9, -3e-9, 4e-9, 5e-9, -1e-9, 2e-9, 3e-9],
+    [0.0, 0.0, 0.0, 0.0, 0.7, 0.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, -0.9],
+    [2.5, 1.0, -3.0, 1.5, -2.0, 0.5, 1.7, -1.1], [0.0] * 8)]
+
+
+@pytest.mark.parametrize("g", G8_SAMPLES)
+def test_eight_generator_exponential_matches_expm(g):
+    """The generators as the reference documents them, checked independently:
+    hermitian, traceless, orthogonal under the trace form; the exponential
+    equals scipy's expm of -i sum g_k lambda_k."""
+    for k, a in enumerate(J8):
+        assert np.allclose(a, a.conj().T) and abs(np.trace(a)) < 1e-15
+        for m, b in enumerate(J8):
+            if m != k:
+                assert abs(np.trace(a @ b)) < 1e-15
+    cols = [np.array([v]) for v in g]
+    e = to_mat(O.matrix_exponential_lie_trotter_eight(cols))[:, :, 0]
+    assert np.abs(e - expm(-1j * hamiltonian(g, "one"))).max() < 1e-12
+    if not g[4:].any():
+        e4 = to_mat(O.matrix_exponential_lie_trotter_one(cols[:4]))[:, :, 0]
+        assert np.abs(e - e4).max() < 1e-13
+
+
+def test_eight_generator_squares_converge():
+    """Fewer squares leave the symmetric splitting error, O(P^-2) at scale
+    1/P = 4^-h: each hypercube divides it by about 16."""
+    g = np.array([0.9, -0.4, 1.3, 0.7, 0.8, -0.6, 0.5, 1.1])
+    cols = [np.array([v]) for v in g]
+    ref = expm(-1j * hamiltonian(g, "one"))
+    errs = [np.abs(to_mat(O.matrix_exponential_lie_trotter_eight(cols, s))[:, :, 0] - ref).max()
+            for s in (0, 2, 4, 6)]
+    assert errs[0] > 1e-3
+    for a, b in zip(errs, errs[1:]):
+        assert a / 20 < b < a / 10
 
 
 def test_lie_trotter_squares_converge():
@@ -121,11 +165,11 @@ def test_gaussian_pulse_known_answer():
 
 
 # ---------------------------------------------------------------- vs an ODE solver
-def _ode_reference(field, spin, t0, t1, dt_out, psi0):
+def _ode_reference(field, spin, t0, t1, dt_out, psi0, lie=4):
     n = int(round((t1 - t0) / dt_out))
 
     def rhs(t, y):
-        f = np.zeros((4, 1))
+        f = np.zeros((lie, 1))
         field.fn(np.array([t]), np.asarray(field.sweep), f)
         return -1j * hamiltonian(f[:, 0] if spin == "one" else f[:3, 0], spin) @ y
 
@@ -137,11 +181,14 @@ def _ode_reference(field, spin, t0, t1, dt_out, psi0):
 
 @pytest.mark.parametrize("field,expo", [(F.TILTED_HALF, "analytic"),
                                         (F.TILTED_HALF, "lie_trotter"),
-                                        (F.TILTED_ONE, None), (F.RABI_SWEEP_ONE, None)])
+                                        (F.TILTED_ONE, None), (F.RABI_SWEEP_ONE, None),
+                                        (F.TILTED_EIGHT, "lie_trotter_8")])
 @pytest.mark.parametrize("rotating", [True, False])
 def test_cf4_matches_ode_solution(field, expo, rotating):
+    """Also pins the eight-generator frame rule (U1/U2 turn at twice the
+    frame rate, V1/V2 at the frame rate, :586-596) through the rotating run."""
     psi0 = O.EIGENSTATES[field.spin]["plus_x"]
-    ts, ref = _ode_reference(field, field.spin, 0.0, 6e-5, 1e-6, psi0)
+    ts, ref = _ode_reference(field, field.spin, 0.0, 6e-5, 1e-6, psi0, 8 if expo == "lie_trotter_8" else 4)
     _, _, state, _ = O.evaluate(field.fn, field.spin, 0.0, 6e-5, 10e-9, 1e-6, psi0, field.sweep,
                                 exponentiation=expo, use_rotating_frame=rotating)
     assert np.abs(state[: ts.size] - ref).max() < 2e-8
@@ -264,12 +311,13 @@ def test_sampler_boundaries():
 
 
 # ---------------------------------------------------------------- two restatements
-@pytest.mark.parametrize("field", [F.TILTED_HALF, F.TILTED_ONE, F.RABI_SWEEP_HALF])
+@pytest.mark.parametrize("field", [F.TILTED_HALF, F.TILTED_ONE, F.RABI_SWEEP_HALF, F.TILTED_EIGHT])
 @pytest.mark.parametrize("method", ["magnus_cf4", "euler", "heun"])
 @pytest.mark.parametrize("rotating", [True, False])
 def test_numpy_and_c_restatements_agree(field, method, rotating):
     dim = 2 if field.spin == "half" else 3
-    expos = ["analytic", "lie_trotter"] if dim == 2 else [None]
+    expos = (["analytic", "lie_trotter"] if dim == 2 else
+             ["lie_trotter_8"] if field is F.TILTED_EIGHT else [None])
     for expo in expos:
         t, u, dt, nf = O.get_time_evolution(field.fn, field.spin, 3e-6, 3e-6 + 5e-5, 1e-7, 1e-6,
                                             field.sweep, exponentiation=expo, integration=method,
