@@ -468,17 +468,44 @@ static __device__ __forceinline__ void ss_gen_dense(const ss_gen& a, ss_cf m[SS_
             m[r][c] = (r == c) ? ss_f(0.f, a.d[r]) : (c == r + 1) ? a.u : (r == c + 1) ? a.l : ss_f(0.f, 0.f);
 }
 
+#if SS_DIM == 2
+// Spin half: G = g.sigma/2 squares to theta^2 I (theta = |g|/2), so exp(A) =
+// cos(theta) I + (sin(theta)/theta) A and the residual is exactly
+// B = (cos theta - 1) I + (sin theta / theta - 1) A.  The two scalars come from
+// their Taylor series in theta^2 (to theta^10; truncation below 1e-11 relative
+// for theta <= 1/2) in float32, or from float64 sincos beyond.
+static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b[SS_D][SS_D])
+{
+    const double th2 = 0.25 * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    float c1, c2;
+    if (th2 <= 0.25) {
+        const float t2 = (float)th2;
+        c1 = t2 * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
+                                4.1666668e-2f), -0.5f);
+        c2 = t2 * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.5052108e-8f, 2.7557319e-6f), -1.9841270e-4f),
+                                8.3333333e-3f), -0.16666667f);
+    } else {
+        const double th = sqrt(th2);
+        double s, c, sh, ch;
+        sincos(th, &s, &c);
+        sincos(0.5 * th, &sh, &ch);
+        c1 = (float)(-2.0 * sh * sh);
+        c2 = (float)(s / th - 1.0);
+    }
+    const ss_gen a = ss_generator(g, 1.0);
+    b[0][0] = ss_f(c1, c2 * a.d[0]);
+    b[1][1] = ss_f(c1, c2 * a.d[1]);
+    b[0][1] = ss_f(c2 * a.u.x, c2 * a.u.y);
+    b[1][0] = ss_f(c2 * a.l.x, c2 * a.l.y);
+}
+#else
 // B = exp(A) - I - A by a Taylor series (order 4 when the norm bound nu of G
 // is at most 0.015, order 6 up to 0.17: truncation below float32 resolution
 // relative to B), with scaling and squaring beyond: the residual X = A + B of
 // A/2^s is squared s times as X <- 2X + X^2, whose first-order part 2A is exact.
 static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b[SS_D][SS_D])
 {
-#if SS_DIM == 2
-    const double nu = 0.5 * (fabs(g[0]) + fabs(g[1]) + fabs(g[2]));
-#else
     const double nu = fabs(g[0]) + fabs(g[1]) + fabs(g[2]) + fabs(g[3]);
-#endif
     int squares = 0;
     double scale = 1.0;
     if (nu > 0.17) {
@@ -536,6 +563,8 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
             for (int c = 0; c < SS_D; ++c) b[r][c] = ss_f(2.f * b[r][c].x + w[r][c].x, 2.f * b[r][c].y + w[r][c].y);
     }
 }
+
+#endif
 
 // P <- exp(A(g)) P for P = I + A(gsum) + b:  gsum += g,
 // b <- b + B + (A(g) + B)(A(gsum_old) + b).

@@ -159,11 +159,11 @@ __global__ void __launch_bounds__(256) spin_kernel(const double* state, double* 
 }
 
 // ---- state scan of one simulation: three passes ---------------------------
-// The latency-bound case (1e5 rows take ~30 us): per-lane prefix products
+// The latency-bound case (1e5 rows take ~25 us): per-lane prefix products
 // within 256-lane workgroups, one workgroup scanning the block totals into
 // block carries, and a pass applying prefix x carry to psi0 per row.
 
-constexpr int kSingleScanChunk = 1;   // matrices per lane
+constexpr int kSingleScanChunk = 2;   // matrices per lane
 constexpr int kSingleScanBlock = 256;
 
 template <int D>

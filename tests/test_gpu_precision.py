@@ -72,6 +72,18 @@ def test_fp32_bulk_large_steps_scaling_and_squaring():
     assert err <= 1e-5
 
 
+def test_fp32_bulk_spin_half_large_angles():
+    """Spin half, lab frame, 700 kHz bias at dt = 500 ns: theta = |g|/2 = 1.1 per
+    step, past the float32 series of the closed-form residual (float64 sincos)."""
+    print("case: fp32_bulk spin-half large angles", flush=True)
+    psi0 = O.EIGENSTATES["half"]["plus_x"]
+    res = _run(F.RABI_SWEEP_HALF, 0.0, 1e-4, 5e-7, 1e-6, psi0, rotating=False)
+    _, st = _oracle(F.RABI_SWEEP_HALF, 0.0, 1e-4, 5e-7, 1e-6, psi0, rotating=False)
+    err = np.abs(res.state - st).max()
+    print(f"  state err {err:.3e}")
+    assert err <= 1e-5
+
+
 def test_fp32_bulk_full_rabi_one_within_1e6():
     """The headline workload (1e6 CF4 steps): state error <= 1e-6 vs float64."""
     print("case: fp32_bulk full rabi_one", flush=True)
