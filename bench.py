@@ -54,11 +54,11 @@ __device__ void get_field(double t, const double* s, double* f)
 # combinations 32 + sampling 14 + time update 1.
 FLOPS_PER_STEP = 2 * (24 * 207 + 143) + 2 * 198 + 24 + 32 + 14 + 1
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per time-evolution launch of the headline workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE counter passes (KiB per dispatch) scaled by what the
-# same counters read for a 256 MiB device copy (fetch x2.0, write x1.0);
-# kept with the profile summaries in profiles/.
-PMC_FETCH_KIB, PMC_WRITE_KIB, FETCH_SCALE, WRITE_SCALE = 154.7, 14843.75, 2.0, 1.0
+# HBM bytes per time-evolution launch of the headline workload (phased launch,
+# one wave per SIMD), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter passes (KiB
+# per dispatch) scaled by what the same counters read for a 256 MiB device
+# copy (fetch x2.0, write x1.0); kept with the profile summaries in profiles/.
+PMC_FETCH_KIB, PMC_WRITE_KIB, FETCH_SCALE, WRITE_SCALE = 176.0, 14843.75, 2.0, 1.0
 TRAFFIC_BYTES = (PMC_FETCH_KIB * FETCH_SCALE + PMC_WRITE_KIB * WRITE_SCALE) * 1024
 
 
