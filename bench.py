@@ -33,6 +33,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+VARIANTS = ("fp32_bulk", "batched", "adjoint", "long_noisy")
 METRIC = "Magnus-CF4 steps/sec on 1 GPU at state err ≤1e-6 vs FP64 oracle; batched/plain"
 
 RABI_ONE = """
@@ -316,6 +317,8 @@ def main():
     ap.add_argument("--lanes", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--variants", default=",".join(VARIANTS),
+                    help="comma-separated subset of the variants to run beside the headline")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32_bulk"),
                     help="arithmetic of the headline run (profiling the other variant)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -326,6 +329,10 @@ def main():
               "integration path on a single device", file=sys.stderr)
         sys.exit(2)
     lanes = args.lanes if args.lanes == "auto" else int(args.lanes)
+    chosen = set() if args.no_variants else {v for v in args.variants.split(",") if v}
+    unknown = chosen - set(VARIANTS)
+    if unknown:
+        ap.error(f"unknown variants {sorted(unknown)}; choose from {list(VARIANTS)}")
 
     import numpy as np
     import torch
@@ -341,14 +348,17 @@ def main():
     probe = fma_probe(torch)
 
     variants = {}
-    if not args.no_variants:
+    if "fp32_bulk" in chosen:
         v32, state32, _ = single_workload(ss, torch, "fp32_bulk", args.steps, args.warmup, lanes)
         variants["fp32_bulk"] = {k: v32[k] for k in ("steps_per_s", "ms_per_step", "kernel_ms",
                                                       "lanes_per_step")}
+    if "batched" in chosen:
         variants["batched_4096_spin_half"] = batched_workload(ss, torch, max(3, args.steps // 4),
                                                               1)
+    if "adjoint" in chosen:
         for prec in ("fp64", "fp32_bulk"):
             variants[f"adjoint_1024_spin_one_{prec}"] = adjoint_workload(ss, torch, prec)
+    if "long_noisy" in chosen:
         variants["long_noisy_1e7_spin_half"] = long_noisy_workload(ss, torch)
 
     if args.copy_probe:
@@ -380,7 +390,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32 residuals / f64",
         "data": "synthetic",
         "config": {"workload": "spin-one Rabi drive, Lie-Trotter (24 squares), rotating frame, "
-                               "plain float64, t 0-100 ms, dt 100 ns, output 1 us",
+                               + ("plain float64" if args.precision == "fp64" else "float32-bulk")
+                               + ", t 0-100 ms, dt 100 ns, output 1 us",
                    "coarse_steps": n, "fine_steps": n * nf,
                    "lanes_per_step": main_run["lanes_per_step"], "parallelism": "single GPU"},
         "state_err_vs_oracle": state_err,
