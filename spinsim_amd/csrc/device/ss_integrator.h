@@ -764,10 +764,50 @@ static __device__ __forceinline__ void ss_group_reduce(ss_cd u[SS_D][SS_D], int 
     }
 }
 
-// Leave the rotating frame and store U and time, :838-852, :818.
+#if SS_PREC == 1
+// Re-unitarisation of a float32-bulk coarse operator: Newton-Schulz steps
+// towards the unitary polar factor, U <- U (3I - U^+ U) / 2 = U - U E / 2 with
+// E = U^+ U - I, in float64.  A step maps a deviation max|E| = e to about
+// 1.5 e^2, so steps repeat (at most three) until e < 1e-8, after which the
+// operator is unitary to float64 rounding and long state products keep their
+// norm.
+static __device__ __forceinline__ void ss_reunitarise(ss_cd u[SS_D][SS_D])
+{
+#pragma unroll 1
+    for (int it = 0; it < 3; ++it) {
+        ss_cd e[SS_D][SS_D];
+        double emax = 0.0;
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c) {
+                ss_cd acc = ss_mul(ss_conj(u[0][r]), u[0][c]);
+#pragma unroll
+                for (int k = 1; k < SS_D; ++k) acc = ss_add(acc, ss_mul(ss_conj(u[k][r]), u[k][c]));
+                if (r == c) acc.x -= 1.0;
+                e[r][c] = acc;
+                emax = fmax(emax, fmax(fabs(acc.x), fabs(acc.y)));
+            }
+        ss_cd ue[SS_D][SS_D];
+        ss_matmul<SS_D>(u, e, ue);
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r)
+#pragma unroll
+            for (int c = 0; c < SS_D; ++c)
+                u[r][c] = ss_c(u[r][c].x - 0.5 * ue[r][c].x, u[r][c].y - 0.5 * ue[r][c].y);
+        if (emax < 1e-8) break;
+    }
+}
+#endif
+
+// Leave the rotating frame and store U and time, :838-852, :818.  Float32-bulk
+// operators are re-unitarised first (once per coarse step).
 static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot, double dt_out, double tc,
                                                 long long row, double* time_out, double* u_out)
 {
+#if SS_PREC == 1
+    ss_reunitarise(u);
+#endif
 #if SS_ROT
     double s, c;
     sincos(rot * dt_out, &s, &c);

@@ -61,6 +61,14 @@ def test_fp32_bulk_lab_frame(field):
     assert err <= 1e-6
 
 
+def _unitarity(u):
+    """max |U U^+ - I| over the coarse operators."""
+    d = u.shape[1]
+    dev = np.abs(np.einsum("nij,nkj->nik", u, u.conj()) - np.eye(d)).max(initial=0)
+    print(f"  max |U U^+ - I| = {dev:.3e}")
+    return dev
+
+
 def test_fp32_bulk_large_steps_scaling_and_squaring():
     """Lab-frame 700 kHz bias: |A| = 0.44 per step, beyond the Taylor range."""
     print("case: fp32_bulk large steps", flush=True)
@@ -70,6 +78,23 @@ def test_fp32_bulk_large_steps_scaling_and_squaring():
     err = np.abs(res.state - st).max()
     print(f"  state err {err:.3e}")
     assert err <= 1e-5
+    assert _unitarity(res.time_evolution) <= 1e-14
+
+
+def test_fp32_bulk_batch_operators_reunitarised():
+    """The batched kernel stores re-unitarised coarse operators too."""
+    import spinsim_amd as ss
+
+    print("case: fp32_bulk batch unitarity", flush=True)
+    rng = np.random.default_rng(5)
+    b = 16
+    sweep = np.tile(np.array(F.RABI_SWEEP_ONE.sweep), (b, 1))
+    sweep[:, 1] *= rng.uniform(0.5, 20.0, b)
+    sim = ss.Simulator(F.RABI_SWEEP_ONE.src, ss.SpinQuantumNumber.ONE, precision="fp32_bulk",
+                       use_rotating_frame=False)
+    res = sim.evaluate_batch(0.0, rng.uniform(1e-5, 5e-5, b), 1e-7, 1e-6,
+                             O.EIGENSTATES["one"]["plus_x"], sweep)
+    assert _unitarity(res.time_evolution.cpu().numpy()) <= 1e-14
 
 
 def test_fp32_bulk_spin_half_large_angles():
@@ -93,11 +118,8 @@ def test_fp32_bulk_full_rabi_one_within_1e6():
     err = np.abs(res.state - st).max()
     print(f"  state err {err:.3e}")
     assert err <= 1e-6
-    # the operators stay unitary to float64 working accuracy on average
-    uu = res.time_evolution
-    dev = np.abs(np.einsum("nij,nkj->nik", uu, uu.conj()) - np.eye(3)).max()
-    print(f"  max |U U^+ - I| = {dev:.3e}")
-    assert dev < 1e-9
+    # every coarse operator is re-unitarised before it is stored
+    assert _unitarity(res.time_evolution) <= 1e-14
 
 
 def test_precision_error_helper():
@@ -107,4 +129,4 @@ def test_precision_error_helper():
     sim = ss.Simulator(F.TILTED_ONE.src, ss.SpinQuantumNumber.ONE, precision="fp32_bulk")
     err = ss.precision_error(sim, 0.0, 4e-4, 1e-7, 1e-6, O.EIGENSTATES["one"]["plus_x"])
     print(f"  {err}")
-    assert 0 < err["state"] <= 1e-8 and err["time_evolution"] <= 1e-8 and err["unitarity"] <= 1e-9
+    assert 0 < err["state"] <= 1e-8 and err["time_evolution"] <= 1e-8 and err["unitarity"] <= 1e-14
