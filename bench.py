@@ -158,6 +158,7 @@ def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
     return {"steps_per_s": plan.fine_steps / (elapsed / steps), "ms_per_step": 1e3 * elapsed / steps,
             "fine_steps": plan.fine_steps, "coarse_rows": plan.total, "simulations": n_sims,
             "lanes_per_step": plan.lanes, "items_per_grab": plan.items_per_grab,
+            "scan_chunk_rows": plan.scan_chunk_rows,
             "state_err_vs_oracle_sampled": max(errs), "sampled_simulations": check,
             "dtype": "f32 residuals / f64 samples, products, scan", "graph": True}
 

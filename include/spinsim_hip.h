@@ -144,22 +144,31 @@ int ss_batch_time_evolution(const ss_module *m, const double *sweep, int64_t swe
                             int items_per_grab, int grid_blocks, ss_stream stream);
 
 /* Workspace bytes ss_state_scan (n rows) and ss_batch_state_scan /
- * ss_batch_costate_scan (n = total rows of the batch) need for dimension dim.
- * A workspace may be reused by calls in stream order; calls that overlap in
- * time need their own. */
+ * ss_batch_costate_scan with the default chunking (chunk_rows 0; n = total
+ * rows of the batch) need for dimension dim.  A workspace may be reused by
+ * calls in stream order; calls that overlap in time need their own. */
 size_t ss_state_scan_workspace(int64_t n, int dim);
+/* Workspace bytes of a batch scan over `total` rows cut into chunks of
+ * chunk_rows rows (0: the default chunking; else a positive multiple of the
+ * 1024-row tile).  0 for invalid arguments.  (No reference counterpart: the
+ * chunk size is a launch parameter of the parallel scan, chosen by the
+ * autotuner.) */
+size_t ss_batch_scan_workspace(int64_t total, int dim, int64_t chunk_rows);
 /* state[0] = psi0, state[i] = u[i-1] state[i-1] for i < n (u[n-1] unused),
- * by a single-pass chained prefix product in float64.  psi0: HOST array of
- * dim complex values; u, state, workspace: device.  spin: NULL, or a device
- * (n, 3) array that also receives the Bloch vectors (as ss_spin) in the same
- * pass. */
+ * as a prefix product in float64 (three passes: lane and workgroup products,
+ * workgroup carries, states).  psi0: HOST array of dim complex values; u,
+ * state, workspace: device.  spin: NULL, or a device (n, 3) array that also
+ * receives the Bloch vectors (as ss_spin) in the last pass. */
 int ss_state_scan(const double *psi0, const double *u, double *state, double *spin, int64_t n,
                   int dim, void *workspace, size_t workspace_bytes, ss_stream stream);
 /* Per-simulation state scan over CSR rows (total = offsets[n_sims]); psi0:
- * device (n_sims, dim); spin: NULL or device (total, 3). */
+ * device (n_sims, dim); spin: NULL or device (total, 3).  chunk_rows: rows per
+ * workgroup chunk, 0 for the default or a positive multiple of 1024; the
+ * workspace holds ss_batch_scan_workspace(total, dim, chunk_rows) bytes. */
 int ss_batch_state_scan(const double *psi0, const double *u, double *state, double *spin,
                         const int64_t *offsets, int64_t n_sims, int64_t total, int dim,
-                        void *workspace, size_t workspace_bytes, ss_stream stream);
+                        int64_t chunk_rows, void *workspace, size_t workspace_bytes,
+                        ss_stream stream);
 
 /* Costates for the adjoint: costate[o+n-1] = lambda_final[s] and
  * costate[i] = u[i+1]^+ costate[i+1] over each simulation's rows, i.e. the
@@ -167,7 +176,8 @@ int ss_batch_state_scan(const double *psi0, const double *u, double *state, doub
  * dim) complex, the loss gradient with respect to psi(T) (2 dL/dpsi*). */
 int ss_batch_costate_scan(const double *lambda_final, const double *u, double *costate,
                           const int64_t *offsets, int64_t n_sims, int64_t total, int dim,
-                          void *workspace, size_t workspace_bytes, ss_stream stream);
+                          int64_t chunk_rows, void *workspace, size_t workspace_bytes,
+                          ss_stream stream);
 /* psi(T) of each simulation: u[last] state[last] (psi0 for a simulation with
  * no rows); out: device (n_sims, dim) complex. */
 int ss_batch_final_state(const double *psi0, const double *u, const double *state,

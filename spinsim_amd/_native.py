@@ -44,10 +44,11 @@ SIGNATURES = {
     "ss_state_scan_workspace": (ctypes.c_size_t, [c_i64, c_int]),
     "ss_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, ctypes.c_size_t,
                               c_vp]),
-    "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp,
-                                    ctypes.c_size_t, c_vp]),
-    "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp,
-                                      ctypes.c_size_t, c_vp]),
+    "ss_batch_scan_workspace": (ctypes.c_size_t, [c_i64, c_int, c_i64]),
+    "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
+                                    c_vp, ctypes.c_size_t, c_vp]),
+    "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
+                                      c_vp, ctypes.c_size_t, c_vp]),
     "ss_batch_final_state": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp]),
     "ss_batch_adjoint": (c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_vp]),
@@ -55,6 +56,10 @@ SIGNATURES = {
     "ss_spin": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
     "ss_fma_probe": (c_int, [c_vp, c_i64, c_int, c_vp]),
 }
+
+
+# Version of the C ABI these signatures describe (ss_abi_version()).
+ABI_VERSION = 5
 
 
 class NativeError(RuntimeError):
@@ -77,6 +82,11 @@ def lib():
                 fn = getattr(handle, name)
                 fn.restype = res
                 fn.argtypes = args
+            version = handle.ss_abi_version()
+            if version != ABI_VERSION:
+                raise NativeError(
+                    f"{LIB_PATH} has C ABI version {version}, this package needs {ABI_VERSION}: "
+                    "rebuild it with `python -m spinsim_amd._build`")
             _lib = handle
     return _lib
 

@@ -161,8 +161,9 @@ class ControlledSimulator:
         L = _native.lib()
         _native.check(L.ss_batch_costate_scan(lam.data_ptr(), plan.u.data_ptr(),
                                               costate.data_ptr(), plan.offsets.data_ptr(), b,
-                                              plan.total, d, plan.scan_ws.data_ptr(),
-                                              plan.scan_ws.numel(), stream),
+                                              plan.total, d, plan.scan_chunk_rows,
+                                              plan.scan_ws.data_ptr(), plan.scan_ws.numel(),
+                                              stream),
                       "ss_batch_costate_scan")
         for c, off in zip(self.controls, self.grad_offsets):
             c.sampler.bind(amps[:, off:off + c.time.size], self.grad_stride)

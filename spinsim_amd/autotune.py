@@ -1,10 +1,11 @@
-"""Launch-shape autotuner for the time-evolution kernels.
+"""Launch-shape autotuner for the time-evolution kernels and the batch scans.
 
 Chooses between the phased default launch and a number of lanes sharing one
-coarse step (and, for batches, how many rows a wave takes per grab of the
-work counter) by timing the candidates on the device with HIP events.
-Results are cached per (kernel, shape, device) in memory and in a JSON file
-next to the code-object cache.
+coarse step (threads per simulation step), for batches how many rows a wave
+takes per grab of the work counter (the tile of work), and the rows per
+workgroup chunk of the per-simulation state scans, by timing the candidates
+on the device with HIP events.  Results are cached per (kernel, shape,
+device) in memory and in a JSON file next to the code-object cache.
 """
 from __future__ import annotations
 
@@ -116,26 +117,63 @@ def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep,
     return best, table
 
 
+def scan_chunk_candidates(total_rows: int, targets=(512, 1024, 2048, 4096, 8192)):
+    """Chunk sizes (whole 1024-row tiles) giving about `targets` chunks."""
+    from .batch import SCAN_TILE_ROWS
+
+    tiles = max(1, -(-int(total_rows) // SCAN_TILE_ROWS))
+    return sorted({SCAN_TILE_ROWS * max(1, -(-tiles // t)) for t in targets})
+
+
+def tune_scan(plan, reps: int = 2, use_cache: bool = True, targets=(512, 1024, 2048, 4096, 8192),
+              warmup: int = 1):
+    """Best rows per chunk of a BatchPlan's state scan; applied to the plan
+    (the costate scan of the adjoint uses the same chunking).  Times the
+    scans over the plan's current operators."""
+    candidates = scan_chunk_candidates(plan.total, targets) if plan.total else [0]
+    key = _key(plan.sim, "scan", plan.n_sims, plan.total, plan.spin_out is not None,
+               tuple(candidates))
+    if use_cache:
+        disk = _load_disk()
+        if key in disk:
+            plan.set_scan_chunk_rows(int(disk[key]))
+            return plan.scan_chunk_rows, None
+    table = {}
+    for c in candidates:
+        plan.set_scan_chunk_rows(c)
+        table[c] = time_launch(plan.run_state_scan, reps=reps, warmup=warmup)
+    best = min(table, key=table.get)
+    plan.set_scan_chunk_rows(best)
+    _store_disk(key, best)
+    return best, table
+
+
 def tune_batch(plan, reps: int = 2, use_cache: bool = True, lanes=None, items=(1, 4, 16),
-               warmup: int = 1):
-    """Best (lanes_per_step, items_per_grab) for a BatchPlan; applied to the plan.
+               warmup: int = 1, scan: bool = True):
+    """Best (lanes_per_step, items_per_grab) for a BatchPlan, and with `scan`
+    the state scan's chunk size (:func:`tune_scan`); applied to the plan.
 
     ``lanes`` defaults to :func:`lane_candidates` of the largest n_fine.
     """
     nf_max = int(plan.n_fine.max().item()) if plan.total else 1
     lanes = tuple(lanes) if lanes is not None else tuple(lane_candidates(nf_max))
     key = _key(plan.sim, "batch", plan.n_sims, plan.total, nf_max, lanes, tuple(items))
+    best = None
     if use_cache:
         disk = _load_disk()
         if key in disk:
-            plan.lanes, plan.items_per_grab = map(int, disk[key])
-            return (plan.lanes, plan.items_per_grab), None
-    table = {}
-    for g in lanes:
-        for it in items:
-            plan.lanes, plan.items_per_grab = g, it
-            table[(g, it)] = time_launch(plan.run_time_evolution, reps=reps, warmup=warmup)
-    best = min(table, key=table.get)
+            best = tuple(map(int, disk[key]))
+    table = None
+    if best is None:
+        table = {}
+        for g in lanes:
+            for it in items:
+                plan.lanes, plan.items_per_grab = g, it
+                table[(g, it)] = time_launch(plan.run_time_evolution, reps=reps, warmup=warmup)
+        best = min(table, key=table.get)
+        _store_disk(key, list(best))
     plan.lanes, plan.items_per_grab = best
-    _store_disk(key, list(best))
+    if scan and plan.total:
+        plan.run_time_evolution()
+        tune_scan(plan, reps=reps, use_cache=use_cache, warmup=warmup)
     return best, table

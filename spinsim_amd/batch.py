@@ -87,6 +87,10 @@ class BatchResults:
                        st.cpu().numpy(), self._plan.sim.spin_calculator, _device_state=st)
 
 
+# Rows of one scan tile (256 threads x 4 rows); chunk sizes are multiples.
+SCAN_TILE_ROWS = 1024
+
+
 class BatchPlan:
     """Device buffers and launch parameters for one batched workload.
 
@@ -145,8 +149,32 @@ class BatchPlan:
         self.grid_blocks = int(grid_blocks)
         self.graph = None
         self.spin_out = (torch.empty((total, 3), dtype=f64, device=dev) if with_spin else None)
-        ws = _native.lib().ss_state_scan_workspace(total, d) if total else 0
-        self.scan_ws = torch.empty(max(int(ws), 1), dtype=torch.uint8, device=dev)
+        self.scan_chunk_rows = 0
+        self.scan_ws = torch.empty(max(self.scan_workspace_bytes(0), 1), dtype=torch.uint8,
+                                   device=dev)
+
+    def scan_workspace_bytes(self, chunk_rows):
+        """Workspace of the state and costate scans at this chunk size (0:
+        default chunking; else a positive multiple of SCAN_TILE_ROWS)."""
+        if not self.total:
+            return 0
+        return int(_native.lib().ss_batch_scan_workspace(self.total, self.sim.dimension,
+                                                         int(chunk_rows)))
+
+    def set_scan_chunk_rows(self, chunk_rows):
+        """Rows per workgroup chunk of the scans; grows the workspace if needed
+        (not while a captured graph uses it)."""
+        chunk_rows = int(chunk_rows)
+        if chunk_rows < 0 or chunk_rows % SCAN_TILE_ROWS:
+            raise ValueError(f"chunk_rows must be 0 or a positive multiple of {SCAN_TILE_ROWS}")
+        need = self.scan_workspace_bytes(chunk_rows)
+        if need > self.scan_ws.numel():
+            if self.graph is not None:
+                raise RuntimeError("the captured graph uses the scan workspace; set the chunk "
+                                   "size before capture()")
+            torch = self.sim._torch()
+            self.scan_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        self.scan_chunk_rows = chunk_rows
 
     def run_time_evolution(self, stream=None):
         torch = self.sim._torch()
@@ -169,7 +197,8 @@ class BatchPlan:
         _native.check(_native.lib().ss_batch_state_scan(
             self.psi0.data_ptr(), self.u.data_ptr(), self.state.data_ptr(),
             None if spin is None else spin.data_ptr(), self.offsets.data_ptr(), self.n_sims,
-            self.total, self.sim.dimension, self.scan_ws.data_ptr(), self.scan_ws.numel(), stream),
+            self.total, self.sim.dimension, self.scan_chunk_rows, self.scan_ws.data_ptr(),
+            self.scan_ws.numel(), stream),
             "ss_batch_state_scan")
 
     def run(self):

@@ -47,7 +47,7 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 4;
+constexpr int kAbiVersion = 5;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
 
@@ -898,15 +898,21 @@ struct ScanLayout {
 // Rows per thread of the state scan; a tile is 256 threads.
 constexpr int kScanRows = 4;
 
-// Chunks of whole tiles, at most about 2048 of them (8 workgroups per CU);
-// the workspace holds one record (map and constant flag) per chunk.
-ScanLayout scan_layout(int64_t n, int dim)
+constexpr long long kScanTileRows = 256LL * kScanRows;
+
+// Chunks of whole tiles: `chunk_rows` rows each (a positive multiple of the
+// 1024-row tile), or by default at most about 2048 chunks (8 workgroups per
+// CU).  The workspace holds one record (map and constant flag) per chunk.
+ScanLayout scan_layout(int64_t n, int dim, int64_t chunk_rows = 0)
 {
     ScanLayout l{};
-    const long long tile = 256LL * kScanRows;
-    l.tiles = (n + tile - 1) / tile;
-    const long long per = std::max(1LL, (l.tiles + 2047) / 2048);
-    l.chunk_rows = tile * per;
+    l.tiles = (n + kScanTileRows - 1) / kScanTileRows;
+    if (chunk_rows > 0) {
+        l.chunk_rows = chunk_rows;
+    } else {
+        const long long per = std::max(1LL, (l.tiles + 2047) / 2048);
+        l.chunk_rows = kScanTileRows * per;
+    }
     l.chunks = (n + l.chunk_rows - 1) / l.chunk_rows;
     l.bytes = align256(size_t(std::max(1LL, l.chunks)) * scan_agg_stride(dim) * 8);
     return l;
@@ -1253,13 +1259,22 @@ size_t ss_state_scan_workspace(int64_t n, int dim)
     return std::max(scan_layout(n, dim).bytes, single_scan_layout(n, dim).bytes);
 }
 
+size_t ss_batch_scan_workspace(int64_t total, int dim, int64_t chunk_rows)
+{
+    if (total <= 0 || (dim != 2 && dim != 3) || chunk_rows < 0 || chunk_rows % kScanTileRows != 0) return 0;
+    return scan_layout(total, dim, chunk_rows).bytes;
+}
+
 namespace {
 
 // Fill the scan arguments from a workspace and launch both passes.
-int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, void* workspace, size_t workspace_bytes,
-                hipStream_t s)
+int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, int64_t chunk_rows, void* workspace,
+                size_t workspace_bytes, hipStream_t s)
 {
-    const ScanLayout l = scan_layout(a.total, dim);
+    if (chunk_rows < 0 || chunk_rows % kScanTileRows != 0)
+        return fail("%s: chunk_rows must be 0 (default) or a positive multiple of %lld, got %lld", who,
+                    kScanTileRows, (long long)chunk_rows);
+    const ScanLayout l = scan_layout(a.total, dim, chunk_rows);
     if (!workspace) return fail("%s: null workspace", who);
     if (workspace_bytes < l.bytes) return fail("%s: workspace of %zu bytes, need %zu", who, workspace_bytes, l.bytes);
     if (l.chunks > 2147483647LL) return fail("%s: too many rows", who);
@@ -1270,7 +1285,7 @@ int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, void* worksp
     // pass 2 with one row per thread once chunks span several tiles (large,
     // bandwidth-bound scans: row-contiguous lanes store coalesced); with
     // kScanRows rows per thread when each chunk is one tile (latency-bound)
-    const bool wide = l.chunk_rows > 256LL * kScanRows;
+    const bool wide = l.chunk_rows > kScanTileRows;
     if (dim == 2 && !reverse) {
         hipLaunchKernelGGL((batch_scan_reduce<2, false, kScanRows>), g, b, 0, s, a);
         if (wide) hipLaunchKernelGGL((batch_scan_apply<2, false, 1>), g, b, 0, s, a);
@@ -1327,8 +1342,8 @@ int ss_state_scan(const double* psi0, const double* u, double* state, double* sp
 }
 
 int ss_batch_state_scan(const double* psi0, const double* u, double* state, double* spin, const int64_t* offsets,
-                        int64_t n_sims, int64_t total, int dim, void* workspace, size_t workspace_bytes,
-                        ss_stream stream)
+                        int64_t n_sims, int64_t total, int dim, int64_t chunk_rows, void* workspace,
+                        size_t workspace_bytes, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_state_scan: dim must be 2 or 3");
     if (n_sims < 0 || total < 0) return fail("ss_batch_state_scan: negative size");
@@ -1344,13 +1359,13 @@ int ss_batch_state_scan(const double* psi0, const double* u, double* state, doub
     a.init = psi0;
     a.total = total;
     a.n_sims = n_sims;
-    return launch_scan("ss_batch_state_scan", a, dim, false, workspace, workspace_bytes,
+    return launch_scan("ss_batch_state_scan", a, dim, false, chunk_rows, workspace, workspace_bytes,
                         static_cast<hipStream_t>(stream));
 }
 
 int ss_batch_costate_scan(const double* lambda_final, const double* u, double* costate, const int64_t* offsets,
-                          int64_t n_sims, int64_t total, int dim, void* workspace, size_t workspace_bytes,
-                          ss_stream stream)
+                          int64_t n_sims, int64_t total, int dim, int64_t chunk_rows, void* workspace,
+                          size_t workspace_bytes, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_costate_scan: dim must be 2 or 3");
     if (n_sims < 0 || total < 0) return fail("ss_batch_costate_scan: negative size");
@@ -1365,7 +1380,7 @@ int ss_batch_costate_scan(const double* lambda_final, const double* u, double* c
     a.init = lambda_final;
     a.total = total;
     a.n_sims = n_sims;
-    return launch_scan("ss_batch_costate_scan", a, dim, true, workspace, workspace_bytes,
+    return launch_scan("ss_batch_costate_scan", a, dim, true, chunk_rows, workspace, workspace_bytes,
                         static_cast<hipStream_t>(stream));
 }
 

@@ -37,7 +37,7 @@ def _reference(u, offsets, init, reverse):
     return out
 
 
-def _run(u, offsets, init, reverse, with_spin=False):
+def _run(u, offsets, init, reverse, with_spin=False, chunk_rows=0):
     import torch
 
     from spinsim_amd import _native
@@ -50,18 +50,18 @@ def _run(u, offsets, init, reverse, with_spin=False):
     dinit = torch.as_tensor(np.ascontiguousarray(init), device="cuda")
     out = torch.full((total, d), float("nan"), dtype=torch.complex128, device="cuda")
     spin = torch.empty((total, 3), dtype=torch.float64, device="cuda") if with_spin else None
-    ws = torch.empty(max(int(L.ss_state_scan_workspace(total, d)), 1), dtype=torch.uint8,
-                     device="cuda")
+    ws = torch.empty(max(int(L.ss_batch_scan_workspace(total, d, chunk_rows)), 1),
+                     dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     b = len(offsets) - 1
     if reverse:
         rc = L.ss_batch_costate_scan(dinit.data_ptr(), du.data_ptr(), out.data_ptr(),
-                                     doff.data_ptr(), b, total, d, ws.data_ptr(), ws.numel(),
-                                     stream)
+                                     doff.data_ptr(), b, total, d, chunk_rows, ws.data_ptr(),
+                                     ws.numel(), stream)
     else:
         rc = L.ss_batch_state_scan(dinit.data_ptr(), du.data_ptr(), out.data_ptr(),
                                    None if spin is None else spin.data_ptr(), doff.data_ptr(), b,
-                                   total, d, ws.data_ptr(), ws.numel(), stream)
+                                   total, d, chunk_rows, ws.data_ptr(), ws.numel(), stream)
     _native.check(rc, "scan")
     return out.cpu().numpy(), (None if spin is None else spin.cpu().numpy())
 
@@ -98,6 +98,27 @@ def test_batch_scan_matches_recurrence(dim, name, reverse):
     u = _unitaries(rng, total, dim)
     init = rng.normal(size=(len(n), dim)) + 1j * rng.normal(size=(len(n), dim))
     got, spin = _run(u, offsets, init, reverse, with_spin=not reverse)
+    ref = _reference(u, offsets, init, reverse)
+    assert np.abs(got - ref).max(initial=0) <= TOL
+    if spin is not None:
+        from oracle import spinsim_oracle as O
+        assert np.abs(spin - O.get_spin(ref)).max(initial=0) <= TOL
+
+
+@pytest.mark.parametrize("chunk_rows", [1024, 3072, 16384])
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("reverse", [False, True], ids=["state", "costate"])
+def test_batch_scan_chunk_sizes(chunk_rows, dim, reverse):
+    """Explicit chunk sizes (the autotuner's candidates): one-tile chunks, and
+    multi-tile chunks with simulations starting anywhere inside them."""
+    print(f"case: scan chunk_rows={chunk_rows} dim={dim} reverse={reverse}", flush=True)
+    rng = np.random.default_rng(14)
+    n = _lengths("mixed", rng)
+    offsets = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    total = int(offsets[-1])
+    u = _unitaries(rng, total, dim)
+    init = rng.normal(size=(len(n), dim)) + 1j * rng.normal(size=(len(n), dim))
+    got, spin = _run(u, offsets, init, reverse, with_spin=not reverse, chunk_rows=chunk_rows)
     ref = _reference(u, offsets, init, reverse)
     assert np.abs(got - ref).max(initial=0) <= TOL
     if spin is not None:
@@ -161,12 +182,13 @@ def test_batch_scan_graph_replays():
     doff = torch.as_tensor(offsets, device="cuda")
     dinit = torch.as_tensor(init, device="cuda")
     out = torch.empty((total, d), dtype=torch.complex128, device="cuda")
-    ws = torch.empty(int(L.ss_state_scan_workspace(total, d)), dtype=torch.uint8, device="cuda")
+    ws = torch.empty(int(L.ss_batch_scan_workspace(total, d, 0)), dtype=torch.uint8,
+                     device="cuda")
 
     def run():
         s = torch.cuda.current_stream().cuda_stream
         _native.check(L.ss_batch_state_scan(dinit.data_ptr(), du.data_ptr(), out.data_ptr(), None,
-                                            doff.data_ptr(), len(n), total, d, ws.data_ptr(),
+                                            doff.data_ptr(), len(n), total, d, 0, ws.data_ptr(),
                                             ws.numel(), s), "scan")
 
     side = torch.cuda.Stream()

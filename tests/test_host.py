@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == 4
+    assert lib.ss_abi_version() == _native.ABI_VERSION == 5
     assert lib.ss_source_hash() != 0
 
 
@@ -49,13 +49,31 @@ def test_library_rejects_bad_arguments():
     assert lib.ss_spin(None, None, -1, 2, None) != 0
     assert lib.ss_state_scan_workspace(0, 2) == 0
     assert lib.ss_state_scan_workspace(100, 3) > 0
-    assert lib.ss_batch_state_scan(None, None, None, None, None, 3, 10, 2, None, 0, None) != 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 3, 10, 2, 0, None, 0, None) != 0
     assert b"null buffer" in lib.ss_last_error()
     assert lib.ss_time_evolution(None, None, 0.0, 1e-7, 1e-6, 10, None, None, None, 65, None) != 0
     assert lib.ss_fma_probe(None, 0, 1, None) != 0
     # zero-sized work is a no-op success, even with null buffers
     assert lib.ss_spin(None, None, 0, 2, None) == 0
-    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, None, 0, None) == 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, None, 0, None) == 0
+
+
+def test_batch_scan_chunk_sizes():
+    """Chunks are whole 1024-row tiles; the workspace holds one record per chunk."""
+    lib = _native.lib()
+    ws = lib.ss_batch_scan_workspace
+    assert ws(5000, 2, 1000) == 0 and ws(5000, 2, -1024) == 0 and ws(5000, 4, 1024) == 0
+    assert ws(0, 2, 0) == 0
+    # 5000 rows in 1024-row chunks: 5 records; in one 8192-row chunk: 1
+    assert ws(5000, 2, 1024) > ws(5000, 2, 8192) > 0
+    assert 0 < ws(10 ** 8, 3, 0) <= lib.ss_state_scan_workspace(10 ** 8, 3)
+    # a chunk size that is not a whole number of tiles is rejected before launch
+    p = ctypes.c_void_p(256)
+    for bad in (1000, -1024, 1536):
+        assert lib.ss_batch_state_scan(p, p, p, None, p, 1, 5000, 2, bad, p, 1 << 20, None) != 0
+        assert b"chunk_rows" in lib.ss_last_error()
+        assert lib.ss_batch_costate_scan(p, p, p, p, 1, 5000, 3, bad, p, 1 << 20, None) != 0
+        assert b"chunk_rows" in lib.ss_last_error()
 
 
 @pytest.mark.parametrize("dim,expo", [(2, 0), (2, 1), (3, 1), (3, 2)])
@@ -270,3 +288,15 @@ def test_simulator_constructs_without_gpu():
     sim = ss.Simulator(field_kitchen_sink, ss.SpinQuantumNumber.HALF,
                        integration_method=ss.IntegrationMethod.HEUN, use_rotating_frame=False)
     assert sim.integration_method.index == 2
+
+
+def test_autotuner_candidates():
+    from spinsim_amd.autotune import lane_candidates, scan_chunk_candidates
+
+    assert lane_candidates(10) == [1, 2, 4, 5, 8, 10]
+    assert lane_candidates(100) == [1, 2, 4, 5, 8, 10, 16, 20, 25, 32, 50, 64]
+    assert lane_candidates(1) == [1]
+    c = scan_chunk_candidates(105_801_688)
+    assert all(x % 1024 == 0 for x in c) and len(c) == 5
+    assert -(-105_801_688 // max(c)) <= 512 and -(-105_801_688 // min(c)) >= 7900
+    assert scan_chunk_candidates(1) == [1024]
