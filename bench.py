@@ -76,6 +76,32 @@ def cpu_baseline(co, psi0, dt, n, nf, cores, reps=3):
     return statistics.median(rates), state
 
 
+def cpu_single_core(co, dt, nf, rows=10000, reps=3):
+    """The C restatement on one host thread over the first `rows` coarse steps
+    (the CPU_SINGLE analogue), median steps/s of reps."""
+    rates = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        co.time_evolution([0.0], T0, dt, DT_OUT, nf, rows, threads=1)
+        rates.append(rows * nf / (time.perf_counter() - t))
+    return statistics.median(rates)
+
+
+def evaluate_end_to_end(ss, torch, lanes, reps=3):
+    """Median ms of Simulator.evaluate() returning numpy arrays (time evolution,
+    scan, device-to-host copies of time, U and state) on the same workload."""
+    sim = ss.Simulator(RABI_ONE, ss.SpinQuantumNumber.ONE)
+    psi0 = ss.SpinQuantumNumber.ONE.plus_z
+    sim.evaluate(T0, T1, DT, DT_OUT, psi0, lanes_per_step=lanes)
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sim.evaluate(T0, T1, DT, DT_OUT, psi0, lanes_per_step=lanes)
+        times.append(1e3 * (time.perf_counter() - t))
+    return statistics.median(times)
+
+
 def single_workload(ss, torch, precision, steps, warmup, lanes):
     """Time K evaluate() passes of the spin-one workload; returns a dict and the state."""
     from spinsim_amd.simulator import derive_steps
@@ -347,6 +373,8 @@ def main():
     n, nf, dt = main_run["coarse"], main_run["n_fine"], main_run["dt"]
     achieved = main_run["fine"] * FLOPS_PER_STEP / (main_run["kernel_ms"] * 1e-3) / 1e12
     probe = fma_probe(torch)
+    e2e_ms = evaluate_end_to_end(ss, torch, main_run["lanes_per_step"]) \
+        if args.precision == "fp64" else None
 
     variants = {}
     if "fp32_bulk" in chosen:
@@ -380,6 +408,12 @@ def main():
         base = {"value": rate, "unit": "steps/s", "cores": cores, "kind": "port",
                 "sample": f"whole workload ({n * nf} CF4 steps: time evolution, state "
                           f"product, spin) on {cores} host threads, median of 3"}
+    base1 = None
+    if not args.no_cpu_baseline:
+        rows = min(n, 10000)
+        base1 = {"value": cpu_single_core(co, dt, nf, rows), "unit": "steps/s", "cores": 1,
+                 "kind": "port", "sample": f"first {rows} coarse steps ({rows * nf} CF4 steps, "
+                                           "time evolution only) on one host thread, median of 3"}
     state_err = float(np.abs(state - ref_state).max())
     if "fp32_bulk" in variants:
         variants["fp32_bulk"]["state_err_vs_oracle"] = float(np.abs(state32 - ref_state).max())
@@ -402,7 +436,9 @@ def main():
                      "traffic": TRAFFIC_BYTES},
         "fp64_fma_probe": {"tflops": probe, "frac_of_published_peak": probe / FP64_PEAK_TFLOPS,
                            "kernel_frac_of_probe": achieved / probe},
+        "evaluate_numpy_ms": e2e_ms,
         "cpu_baseline": base,
+        "cpu_baseline_1core": base1,
         "variants": variants,
     }
     print(json.dumps(line), flush=True)

@@ -388,6 +388,8 @@ class Simulator:
                  time_step_output: float, state_init, sweep_parameters=[0], *,
                  output: str = "numpy", lanes_per_step: int = None) -> Results:
         """Integrate and return the state time series (spinsim/__init__.py:993-1085)."""
+        if output not in ("numpy", "torch"):
+            raise ValueError("output must be 'numpy' or 'torch'")
         torch = self._torch()
         dt, n, ends = derive_steps(time_start, time_end, time_step_integration, time_step_output)
         if n < 0:
@@ -401,8 +403,6 @@ class Simulator:
         state = self.state_scan(state_init, u)
         if output == "torch":
             return Results(time, u, state, self.spin_calculator)
-        if output != "numpy":
-            raise ValueError("output must be 'numpy' or 'torch'")
         return Results(time.cpu().numpy(), u.cpu().numpy(), state.cpu().numpy(),
                        self.spin_calculator, _device_state=state)
 
