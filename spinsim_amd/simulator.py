@@ -110,6 +110,11 @@ class FieldSource:
         raise TypeError("get_field must be a Python function, HIP-C source text or a FieldSource")
 
 
+# Largest evaluate() result (time, U and state) staged through pinned host
+# memory; larger results are copied to pageable memory directly.
+PINNED_RESULT_BYTES = 1 << 31
+
+
 class Results:
     """The results of an evaluation (spinsim/__init__.py:271-332).
 
@@ -400,11 +405,37 @@ class Simulator:
         u = torch.empty((n, d, d), dtype=torch.complex128, device=dev)
         self._get_time_evolution(sweep_parameters, time, ends, dt, time_step_output, u,
                                  lanes_per_step=lanes_per_step)
-        state = self.state_scan(state_init, u)
         if output == "torch":
-            return Results(time, u, state, self.spin_calculator)
-        return Results(time.cpu().numpy(), u.cpu().numpy(), state.cpu().numpy(),
-                       self.spin_calculator, _device_state=state)
+            return Results(time, u, self.state_scan(state_init, u), self.spin_calculator)
+        # numpy results: time and U go to pinned host memory on a side stream
+        # while the state scan runs, then the states follow
+        pinned = n * (d * d + d + 1) * 16 <= PINNED_RESULT_BYTES
+        if not pinned:
+            state = self.state_scan(state_init, u)
+            return Results(time.cpu().numpy(), u.cpu().numpy(), state.cpu().numpy(),
+                           self.spin_calculator, _device_state=state)
+        main = torch.cuda.current_stream(dev)
+        side = self._copy_stream(dev)
+        h_time = torch.empty(time.shape, dtype=time.dtype, pin_memory=True)
+        h_u = torch.empty(u.shape, dtype=u.dtype, pin_memory=True)
+        h_state = torch.empty((n, d), dtype=u.dtype, pin_memory=True)
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            h_time.copy_(time, non_blocking=True)
+            h_u.copy_(u, non_blocking=True)
+        state = self.state_scan(state_init, u)
+        h_state.copy_(state, non_blocking=True)
+        main.synchronize()
+        side.synchronize()
+        return Results(h_time.numpy(), h_u.numpy(), h_state.numpy(), self.spin_calculator,
+                       _device_state=state)
+
+    def _copy_stream(self, device):
+        """A side stream (default priority) per device for result copies."""
+        streams = self.__dict__.setdefault("_copy_streams", {})
+        if device.index not in streams:
+            streams[device.index] = self._torch().cuda.Stream(device)
+        return streams[device.index]
 
     def evaluate_batch(self, time_start, time_end, time_step_integration, time_step_output,
                        state_init, sweep_parameters, **kw):

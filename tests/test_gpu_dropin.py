@@ -113,3 +113,38 @@ def test_kernel_level_entry_points_and_device_outputs():
     assert np.abs(sim.spin_calculator(st_ref) - co.get_spin(st_ref)).max() <= 1e-12
     info = sim.module_info()
     assert info["vgprs"] > 0 and info["scratch_bytes"] == 0
+
+
+def test_numpy_results_staged_through_pinned_memory():
+    """evaluate() numpy results (pinned staging, copies overlapping the scan)
+    equal the device results bit for bit, and later calls never overwrite
+    earlier results; above the staging limit the plain copy path gives the
+    same arrays."""
+    import torch
+
+    import spinsim_amd as ss
+    from spinsim_amd import simulator as S
+
+    print("case: pinned numpy results", flush=True)
+    sim = ss.Simulator(F.TILTED_ONE.src, ss.SpinQuantumNumber.ONE)
+    args = (0.0, 3e-4, 1e-7, 1e-6)
+    dev = sim.evaluate(*args, O.EIGENSTATES["one"]["plus_x"], output="torch")
+    a = sim.evaluate(*args, O.EIGENSTATES["one"]["plus_x"])
+    keep = a.state.copy()
+    for name in ("time", "time_evolution", "state"):
+        assert np.array_equal(getattr(a, name), getattr(dev, name).cpu().numpy()), name
+    b = sim.evaluate(*args, O.EIGENSTATES["one"]["minus_z"])
+    torch.cuda.synchronize()
+    assert np.array_equal(a.state, keep)
+    assert not np.array_equal(a.state, b.state)
+    assert np.array_equal(a.spin, dev.spin.cpu().numpy())
+    limit = S.PINNED_RESULT_BYTES
+    try:
+        S.PINNED_RESULT_BYTES = 0
+        c = sim.evaluate(*args, O.EIGENSTATES["one"]["plus_x"])
+    finally:
+        S.PINNED_RESULT_BYTES = limit
+    for name in ("time", "time_evolution", "state"):
+        assert np.array_equal(getattr(a, name), getattr(c, name)), name
+    empty = sim.evaluate(0.0, 0.0, 1e-7, 1e-6, O.EIGENSTATES["one"]["plus_x"])
+    assert empty.state.shape == (0, 3)
