@@ -352,7 +352,7 @@ def main():
                     help="also run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); spinsim_amd runs the "
               "integration path on a single device", file=sys.stderr)
         sys.exit(2)
     lanes = args.lanes if args.lanes == "auto" else int(args.lanes)
