@@ -15,6 +15,10 @@ import os
 import statistics
 import threading
 
+# Phased launches timed by tune_lanes: the resident waves, and 1-4 waves per
+# SIMD (fewer, longer items against more waves to hide latency).
+PHASED_CANDIDATES = [0, -1, -2, -3, -4]
+
 _lock = threading.Lock()
 _memo: dict = {}
 
@@ -82,14 +86,36 @@ def time_launch(fn, reps=3, warmup=1):
     return statistics.median(times)
 
 
-def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep, t0: float = 0.0,
-               reps: int = 3, use_cache: bool = True):
-    """Best lanes_per_step for one simulation of this shape (0 / -1: phased
-    launch on the resident waves / one wave per SIMD, else lanes per coarse
-    step); returns (lanes, table)."""
+def time_interleaved(fns: dict, rounds: int = 5, warmup: int = 1):
+    """Median milliseconds of each fns[key]() on the current stream, the
+    candidates timed in turn within every round so that slow drifts of the
+    device clock affect them alike."""
     import torch
 
-    candidates = [0, -1] + lane_candidates(n_fine)
+    for _ in range(warmup):
+        for fn in fns.values():
+            fn()
+    times = {k: [] for k in fns}
+    for _ in range(rounds):
+        for k, fn in fns.items():
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            times[k].append(a.elapsed_time(b))
+    return {k: statistics.median(v) for k, v in times.items()}
+
+
+def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep, t0: float = 0.0,
+               reps: int = 7, use_cache: bool = True):
+    """Best lanes_per_step for one simulation of this shape (0: phased launch
+    on the resident waves, -k: phased on k waves per SIMD for k = 1..4, else
+    lanes per coarse step); returns (lanes, table)."""
+    import torch
+
+    candidates = PHASED_CANDIDATES + lane_candidates(n_fine)
     key = _key(sim, "single", n_coarse, n_fine, tuple(candidates))
     if use_cache:
         with _lock:
@@ -106,10 +132,9 @@ def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep,
     u = torch.empty((n_coarse, d, d), dtype=torch.complex128, device=dev)
     sw = sweep if isinstance(sweep, torch.Tensor) else torch.as_tensor(sweep, dtype=torch.float64,
                                                                         device=dev)
-    table = {}
-    for g in candidates:
-        table[g] = time_launch(
-            lambda: sim.launch_time_evolution(sw, t0, dt, dt_out, time_out, u, g), reps=reps)
+    table = time_interleaved(
+        {g: (lambda g=g: sim.launch_time_evolution(sw, t0, dt, dt_out, time_out, u, g))
+         for g in candidates}, rounds=reps)
     best = min(table, key=table.get)
     with _lock:
         _memo[key] = best
@@ -138,10 +163,14 @@ def tune_scan(plan, reps: int = 2, use_cache: bool = True, targets=(512, 1024, 2
         if key in disk:
             plan.set_scan_chunk_rows(int(disk[key]))
             return plan.scan_chunk_rows, None
-    table = {}
-    for c in candidates:
-        plan.set_scan_chunk_rows(c)
-        table[c] = time_launch(plan.run_state_scan, reps=reps, warmup=warmup)
+    # one workspace large enough for every candidate (the smallest chunks)
+    plan.set_scan_chunk_rows(min(candidates))
+
+    def run(c):
+        plan.scan_chunk_rows = c
+        plan.run_state_scan()
+    table = time_interleaved({c: (lambda c=c: run(c)) for c in candidates}, rounds=reps,
+                             warmup=warmup)
     best = min(table, key=table.get)
     plan.set_scan_chunk_rows(best)
     _store_disk(key, best)

@@ -19,14 +19,14 @@ def test_tune_lanes_single_simulation():
     import torch
 
     import spinsim_amd as ss
-    from spinsim_amd.autotune import lane_candidates, tune_lanes
+    from spinsim_amd.autotune import PHASED_CANDIDATES, lane_candidates, tune_lanes
 
     print("case: tune_lanes rabi_one 2 ms", flush=True)
     sim = ss.Simulator(F.RABI_ONE.src, ss.SpinQuantumNumber.ONE)
     dt, n, nf, _ = O.timing(0.0, 2e-3, 100e-9, 1e-6)
     sweep = torch.zeros(1, dtype=torch.float64, device="cuda")
     lanes, table = tune_lanes(sim, n, nf, dt, 1e-6, sweep, 0.0, reps=1, use_cache=False)
-    assert set(table) == {0, -1, *lane_candidates(nf)}
+    assert set(table) == {*PHASED_CANDIDATES, *lane_candidates(nf)}
     assert lanes in table and all(t > 0 for t in table.values())
     res = sim.evaluate(0.0, 2e-3, 100e-9, 1e-6, O.EIGENSTATES["one"]["plus_z"],
                        lanes_per_step=lanes)
