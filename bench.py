@@ -55,10 +55,11 @@ __device__ void get_field(double t, const double* s, double* f)
 # combinations 32 + sampling 14 + time update 1.
 FLOPS_PER_STEP = 2 * (24 * 207 + 143) + 2 * 198 + 24 + 32 + 14 + 1
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per time-evolution launch of the headline workload (phased launch,
-# one wave per SIMD), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter passes (KiB
-# per dispatch) scaled by what the same counters read for a 256 MiB device
-# copy (fetch x2.0, write x1.0); kept with the profile summaries in profiles/.
+# HBM bytes per time-evolution launch of the headline workload (phased launch
+# on one and on three waves per SIMD read the same), from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE counter passes (KiB per dispatch) scaled by what the same counters
+# read for a 256 MiB device copy (fetch x2.0, write x1.0); kept with the
+# profile summaries in profiles/.
 PMC_FETCH_KIB, PMC_WRITE_KIB, FETCH_SCALE, WRITE_SCALE = 176.0, 14843.75, 2.0, 1.0
 TRAFFIC_BYTES = (PMC_FETCH_KIB * FETCH_SCALE + PMC_WRITE_KIB * WRITE_SCALE) * 1024
 
@@ -118,9 +119,10 @@ def single_workload(ss, torch, precision, steps, warmup, lanes):
     spin = torch.empty((n, 3), dtype=torch.float64, device=dev)
     ws = sim.scan_workspace(n, dev)
     stream = torch.cuda.current_stream(dev)
+    table = None
     if lanes == "auto":
         from spinsim_amd.autotune import tune_lanes
-        lanes, _ = tune_lanes(sim, n, nf, dt, DT_OUT, sweep, T0)
+        lanes, table = tune_lanes(sim, n, nf, dt, DT_OUT, sweep, T0)
 
     def step(events=None):
         if events is not None:
@@ -143,6 +145,7 @@ def single_workload(ss, torch, precision, steps, warmup, lanes):
     kernel_ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
     out = {"steps_per_s": n * nf / (elapsed / steps), "ms_per_step": 1e3 * elapsed / steps,
            "kernel_ms": kernel_ms, "lanes_per_step": lanes, "coarse": n, "fine": n * nf,
+           "autotune_ms": None if table is None else {str(k): v for k, v in table.items()},
            "dt": dt, "n_fine": nf}
     return out, state.cpu().numpy(), psi0
 
@@ -429,6 +432,7 @@ def main():
                                + ", t 0-100 ms, dt 100 ns, output 1 us",
                    "coarse_steps": n, "fine_steps": n * nf,
                    "lanes_per_step": main_run["lanes_per_step"], "parallelism": "single GPU"},
+        "autotune_ms": main_run["autotune_ms"],
         "state_err_vs_oracle": state_err,
         "time_evolution_kernel_ms": main_run["kernel_ms"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
