@@ -130,6 +130,13 @@ int ss_module_info(const ss_module *m, int *vgprs, int *sgprs, int *scratch_byte
 int ss_time_evolution(const ss_module *m, const double *sweep, double t0, double dt,
                       double dt_out, int64_t n_coarse, double *time_out, double *u_out,
                       const ss_table *tables, int lanes_per_step, ss_stream stream);
+/* The phase table a phased ss_time_evolution launch on `waves` waves uses
+ * (host only): *n_phases phases; phase p starts at coarse row first_row[p]
+ * and has items[p] items of 64/lanes[p] rows, each row's fine steps split
+ * evenly between lanes[p] lanes.  Arrays hold 8 entries.  (No reference
+ * counterpart: a launch-planning query.) */
+int ss_time_evolution_plan(int64_t n_coarse, int n_fine, int64_t waves, int *n_phases,
+                           int *lanes, int64_t *first_row, int64_t *items);
 
 /* Batched time evolution: simulation s owns rows [offsets[s], offsets[s+1])
  * of time_out / u_out and reads sweep row s (sweep + s*sweep_stride).  All

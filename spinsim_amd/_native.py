@@ -45,6 +45,7 @@ SIGNATURES = {
     "ss_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, ctypes.c_size_t,
                               c_vp]),
     "ss_batch_scan_workspace": (ctypes.c_size_t, [c_i64, c_int, c_i64]),
+    "ss_time_evolution_plan": (c_int, [c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
                                     c_vp, ctypes.c_size_t, c_vp]),
     "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
@@ -59,7 +60,7 @@ SIGNATURES = {
 
 
 # Version of the C ABI these signatures describe (ss_abi_version()).
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 
 class NativeError(RuntimeError):

@@ -47,7 +47,7 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 5;
+constexpr int kAbiVersion = 6;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
 
@@ -1115,7 +1115,9 @@ static void sample_offsets(int method, double dt, double* off_a, double* off_b)
 // Whole rounds (one item per wave) of the coarser G are taken while they fit
 // the ideal per-lane budget N n_fine / (64 waves); the remaining rows go to
 // the finest G, whose items cost the fewest steps, so waves finish within
-// about one such item of each other.
+// about one such item of each other.  (Splitting the finest items further, at
+// half steps, measured no faster: the last item's wave is not what bounds the
+// launch.)
 static void plan_phases(long long n_coarse, int n_fine, long long waves, ss_te_args* a)
 {
     std::vector<int> cand;
@@ -1153,6 +1155,23 @@ static void plan_phases(long long n_coarse, int n_fine, long long waves, ss_te_a
     a->items = 0;
     for (int p = 0; p < n; ++p) a->items += a->phase_items[p];
     a->waves = std::max(1LL, std::min(waves, a->items));
+}
+
+int ss_time_evolution_plan(int64_t n_coarse, int n_fine, int64_t waves, int* n_phases, int* lanes,
+                           int64_t* first_row, int64_t* items)
+{
+    if (!n_phases || !lanes || !first_row || !items) return fail("ss_time_evolution_plan: null buffer");
+    if (n_coarse <= 0 || n_fine < 0 || waves <= 0) return fail("ss_time_evolution_plan: bad sizes");
+    ss_te_args a;
+    std::memset(&a, 0, sizeof(a));
+    plan_phases(n_coarse, n_fine, waves, &a);
+    *n_phases = a.n_phases;
+    for (int p = 0; p < a.n_phases; ++p) {
+        lanes[p] = a.phase_lanes[p];
+        first_row[p] = a.phase_row[p];
+        items[p] = a.phase_items[p];
+    }
+    return 0;
 }
 
 int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double dt, double dt_out,

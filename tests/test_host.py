@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == _native.ABI_VERSION == 5
+    assert lib.ss_abi_version() == _native.ABI_VERSION == 6
     assert lib.ss_source_hash() != 0
 
 
@@ -300,3 +300,58 @@ def test_autotuner_candidates():
     assert all(x % 1024 == 0 for x in c) and len(c) == 5
     assert -(-105_801_688 // max(c)) <= 512 and -(-105_801_688 // min(c)) >= 7900
     assert scan_chunk_candidates(1) == [1024]
+
+
+
+def _plan(n_coarse, n_fine, waves):
+    lib = _native.lib()
+    n = ctypes.c_int()
+    lanes = (ctypes.c_int * 8)()
+    rows, items = (ctypes.c_int64 * 8)(), (ctypes.c_int64 * 8)()
+    _native.check(lib.ss_time_evolution_plan(n_coarse, n_fine, waves, ctypes.byref(n), lanes,
+                                             rows, items), "plan")
+    return [(lanes[p], rows[p], items[p]) for p in range(n.value)]
+
+
+def _lane_steps(n_fine, lanes):
+    """Fine steps each rank of a group integrates, as the kernel derives them
+    (contiguous runs, the first n_fine % lanes ranks one longer)."""
+    base, rem = divmod(n_fine, lanes)
+    out = []
+    for rank in range(lanes):
+        j0 = rank * base + min(rank, rem)
+        out.append(list(range(j0, j0 + base + (1 if rank < rem else 0))))
+    return out
+
+
+@pytest.mark.parametrize("n_coarse,n_fine,waves", [
+    (100000, 10, 3072), (100000, 10, 1024), (100000, 10, 2048), (200000, 5, 1024),
+    (100000, 100, 1024), (7, 3, 4096), (1, 1, 1024), (12345, 37, 777), (100000, 0, 1024),
+    (50000, 32, 3072), (10 ** 7, 1, 4096)])
+def test_phase_plan_covers_every_row_and_step(n_coarse, n_fine, waves):
+    plan = _plan(n_coarse, n_fine, waves)
+    assert 1 <= len(plan) <= 8
+    row = 0
+    budget = n_coarse * n_fine / (64 * waves)
+    cost = 0
+    for p, (lanes, first, items) in enumerate(plan):
+        assert first == row and 1 <= lanes <= 64 and items > 0
+        last = p == len(plan) - 1
+        assert last or items % waves == 0       # whole rounds before the last phase
+        row = min(n_coarse, first + items * (64 // lanes)) if last else first + items * (64 // lanes)
+        steps = _lane_steps(n_fine, lanes)
+        assert [j for r in steps for j in r] == list(range(n_fine))
+        cost += -(-items // waves) * max(len(r) for r in steps)
+    assert row == n_coarse
+    # every wave ends within one finest item of its ideal share (counting the
+    # lanes a group of the finest size leaves idle)
+    g = plan[-1][0]
+    used = (64 // g) * g / 64
+    assert cost <= math.ceil(budget / used) + max(len(r) for r in _lane_steps(n_fine, g)) + 1
+
+
+def test_phase_plan_of_the_spin_one_workload():
+    """1e5 rows of 10 steps on three waves per SIMD: one round of two lanes per
+    row (5 steps each), the remaining 1696 rows at ten lanes per row."""
+    plan = _plan(100000, 10, 3072)
+    assert plan == [(2, 0, 3072), (10, 3072 * 32, -(-(100000 - 3072 * 32) // 6))]
