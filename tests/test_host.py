@@ -355,3 +355,15 @@ def test_phase_plan_of_the_spin_one_workload():
     row (5 steps each), the remaining 1696 rows at ten lanes per row."""
     plan = _plan(100000, 10, 3072)
     assert plan == [(2, 0, 3072), (10, 3072 * 32, -(-(100000 - 3072 * 32) // 6))]
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built library every entry point raises; nothing falls back
+    to a CPU path."""
+    monkeypatch.setattr(_native, "LIB_PATH", str(tmp_path / "libspinsim_hip.so"))
+    monkeypatch.setattr(_native, "_lib", None)
+    with pytest.raises(_native.NativeError, match="no CPU fallback"):
+        _native.lib()
+    with pytest.raises(_native.NativeError):
+        ss.Simulator(F.LARMOR.src + "\n// variant not in the code cache\n",
+                     ss.SpinQuantumNumber.HALF)
