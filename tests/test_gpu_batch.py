@@ -116,3 +116,71 @@ def test_batch_graph_replays_match_eager():
         # unrelated eager work between replays
         x = torch.randn(1000, device="cuda")
         _ = (x * 2).sum().item()
+
+
+RANDOM_FIELD_ONE = """
+__device__ void get_field(double t, const double* s, double* f)
+{ f[0] = s[0] * cos(s[1] * t + s[2]); f[1] = s[3] * sin(s[4] * t);
+  f[2] = s[5] + s[6] * cos(s[7] * t); f[3] = s[8] * cos(s[9] * t); }
+"""
+RANDOM_FIELD_HALF = """
+__device__ void get_field(double t, const double* s, double* f)
+{ f[0] = s[0] * cos(s[1] * t + s[2]); f[1] = s[3] * sin(s[4] * t);
+  f[2] = s[5] + s[6] * cos(s[7] * t); }
+"""
+
+
+@pytest.mark.parametrize("k", range(12))
+def test_random_batch_matches_oracle(k):
+    """Seeded random batches: per-simulation start, duration, integration and
+    output steps and sweep rows; random lanes per step and rows per grab;
+    float64 to 1e-9, float32-bulk to 1e-6 on the states."""
+    import spinsim_amd as ss
+
+    rng = np.random.default_rng(500 + k)
+    spin = "one" if k % 2 else "half"
+    method = ("magnus_cf4", "euler", "heun")[k % 3]
+    precision = "fp32_bulk" if k % 4 == 3 else "fp64"
+    rotating = bool(rng.integers(2)) if precision == "fp64" else True
+    b = int(rng.integers(1, 70))
+    tau = 2 * math.pi
+    amp = tau * 10 ** rng.uniform(3, 4.5, b)
+    sweep = np.stack([amp, tau * 10 ** rng.uniform(3, 5.5, b), rng.uniform(0, tau, b),
+                      amp * rng.uniform(0, 1, b), tau * 10 ** rng.uniform(3, 5, b),
+                      tau * 10 ** rng.uniform(4, 5.5, b), amp * rng.uniform(0, 0.3, b),
+                      tau * 10 ** rng.uniform(2, 4, b), amp * rng.uniform(0, 0.5, b),
+                      tau * 10 ** rng.uniform(3, 5, b)], 1)
+    t0 = rng.uniform(-5e-5, 5e-5, b)
+    dt = 10 ** rng.uniform(-8, -6.8, b)
+    dt_out = dt * rng.integers(1, 30, b)
+    t1 = t0 + dt_out * rng.integers(0, 400, b) + dt_out * rng.uniform(0, 0.99, b)
+    d = 2 if spin == "half" else 3
+    psi = rng.normal(size=(b, d)) + 1j * rng.normal(size=(b, d))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    lanes = int(rng.choice([1, 2, 3, 4, 8, 16, 32]))
+    items = int(rng.choice([1, 2, 4, 16]))
+    print(f"case {k}: spin={spin} {method} {precision} rotating={rotating} B={b} "
+          f"lanes={lanes} items={items}", flush=True)
+    src = RANDOM_FIELD_HALF if spin == "half" else RANDOM_FIELD_ONE
+    sq = ss.SpinQuantumNumber.HALF if spin == "half" else ss.SpinQuantumNumber.ONE
+    methods = {"magnus_cf4": "MAGNUS_CF4", "euler": "MIDPOINT_SAMPLE", "heun": "HALF_STEP"}
+    sim = ss.Simulator(src, sq, use_rotating_frame=rotating, precision=precision,
+                       integration_method=getattr(ss.IntegrationMethod, methods[method]))
+    res = sim.evaluate_batch(t0, t1, dt, dt_out, psi, sweep, lanes_per_step=lanes,
+                             items_per_grab=items)
+    co = COracle(src, d, method, None, rotating)
+    tol = TOL if precision == "fp64" else 1e-6
+    spin_all = res.spin.cpu().numpy()
+    worst = 0.0
+    for s in range(b):
+        r = res[s]
+        t, u, st = _oracle_sim(co, sweep[s], t0[s], t1[s], dt[s], dt_out[s], psi[s])
+        assert np.array_equal(r.time, t), s
+        err = np.abs(r.state - st).max(initial=0)
+        worst = max(worst, err)
+        assert err <= tol, s
+        if precision == "fp64":
+            assert np.abs(r.time_evolution - u).max(initial=0) <= TOL, s
+            a, e = res.offsets[s], res.offsets[s + 1]
+            assert np.abs(spin_all[a:e] - co.get_spin(st)).max(initial=0) <= TOL, s
+    print(f"  max state error {worst:.2e} over {b} simulations")
