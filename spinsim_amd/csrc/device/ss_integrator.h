@@ -1034,31 +1034,36 @@ static __device__ __forceinline__ double ss_re_dot(const ss_cd a[SS_D], const ss
     return s;
 }
 
-// Accumulate grad[k] += w * Re(lam^+ L(A, A(e_k)) psi) for the axes in mask.
+// Accumulate grad[k] += w * Re(lam^+ L(A, A(e_k)) psi) for the axes in mask:
+//     Re(lam^+ L(A, E) psi) = sum_{p+q<=N} c_{p+q} Re(alpha_p^+ E beta_q),
+// alpha_p = (A^+)^p lam = (-A)^p lam, beta_q = A^q psi, c_n = 1/(n+1)!; both
+// power sequences once (2N generator products), then for each q the
+// combination gamma_q = sum_p c_{p+q} alpha_p (vector sums only).
 static __device__ __forceinline__ void ss_frechet_grad(const ss_gen64& a, const ss_cd lam[SS_D],
                                                        const ss_cd psi[SS_D], unsigned mask, double w,
                                                        double grad[SS_LIE])
 {
-    // beta_q = A^q psi
-    ss_cd beta[SS_ADJ_ORDER + 1][SS_D];
+    ss_cd beta[SS_ADJ_ORDER + 1][SS_D], alpha[SS_ADJ_ORDER + 1][SS_D];
 #pragma unroll
-    for (int r = 0; r < SS_D; ++r) beta[0][r] = psi[r];
+    for (int r = 0; r < SS_D; ++r) {
+        beta[0][r] = psi[r];
+        alpha[0][r] = lam[r];
+    }
 #pragma unroll
-    for (int q = 1; q <= SS_ADJ_ORDER; ++q) ss_gen_vec(a, 1.0, beta[q - 1], beta[q]);
-    // gamma_q = sum_p c_{p+q} (-A)^p lam, c_n = 1/(n+1)!, by Horner
+    for (int q = 1; q <= SS_ADJ_ORDER; ++q) {
+        ss_gen_vec(a, 1.0, beta[q - 1], beta[q]);
+        ss_gen_vec(a, -1.0, alpha[q - 1], alpha[q]);
+    }
     const double c[SS_ADJ_ORDER + 1] = {1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
 #pragma unroll
     for (int q = 0; q <= SS_ADJ_ORDER; ++q) {
         ss_cd gam[SS_D];
 #pragma unroll
-        for (int r = 0; r < SS_D; ++r) gam[r] = ss_scale(lam[r], c[SS_ADJ_ORDER]);
+        for (int r = 0; r < SS_D; ++r) gam[r] = ss_scale(alpha[0][r], c[q]);
 #pragma unroll
-        for (int p = SS_ADJ_ORDER - q - 1; p >= 0; --p) {
-            ss_cd t[SS_D];
-            ss_gen_vec(a, -1.0, gam, t);
+        for (int p = 1; p <= SS_ADJ_ORDER - q; ++p)
 #pragma unroll
-            for (int r = 0; r < SS_D; ++r) gam[r] = ss_add(t[r], ss_scale(lam[r], c[p + q]));
-        }
+            for (int r = 0; r < SS_D; ++r) gam[r] = ss_add(gam[r], ss_scale(alpha[p][r], c[p + q]));
 #pragma unroll
         for (int k = 0; k < SS_LIE; ++k) {
             if (!(mask & (1u << k))) continue;
