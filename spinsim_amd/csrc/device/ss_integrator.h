@@ -1189,15 +1189,15 @@ static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& 
     }
 }
 
-// dL/dg of the rotating-frame samples -> lab frame (x, y rotate back; z, q unchanged).
-static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], double rot, double t_rel, double out[4])
+// dL/dg of the rotating-frame samples -> lab frame (x, y rotate back; z, q
+// unchanged); w is the sample's winding exp(i rot t_rel).
+static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], ss_cd w, double out[4])
 {
     out[0] = drot[0];
     out[1] = drot[1];
     out[2] = drot[2];
     out[3] = SS_LIE > 3 ? drot[3] : 0.0;
 #if SS_ROT
-    const ss_cd w = ss_winding(rot, t_rel);
 #if SS_DIM == 2
     const ss_cd wf = ss_mul(w, w);
 #else
@@ -1207,8 +1207,7 @@ static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], dou
     out[0] = gw.x;
     out[1] = gw.y;
 #else
-    (void)rot;
-    (void)t_rel;
+    (void)w;
 #endif
 }
 
@@ -1296,7 +1295,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
         ss_adjoint_exp(g1, psi, lam, mask, d1, a.status);
 #pragma unroll
         for (int k = 0; k < SS_LIE; ++k) dfa[k] = dt * d1[k];
-        ss_unframe(dfa, rot, tra, lab);
+        ss_unframe(dfa, wa, lab);
         ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
 #else
         double fb[SS_FIELD_LEN], g2[SS_FIELD_LEN], d2[SS_LIE], dfb[SS_LIE];
@@ -1333,9 +1332,9 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
             dfb[k] = 0.5 * dt * d2[k];
         }
 #endif
-        ss_unframe(dfa, rot, tra, lab);
+        ss_unframe(dfa, wa, lab);
         ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
-        ss_unframe(dfb, rot, trb, lab);
+        ss_unframe(dfb, wb, lab);
         ss_scatter_sample(a, &ctx, trb + tc, lab, win, base, grad_row);
 #endif
     }
