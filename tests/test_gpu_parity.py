@@ -82,6 +82,33 @@ def test_larmor_full_size_and_closed_form():
 TAU1000 = 2 * math.pi * 1000
 
 
+def test_larmor_twenty_million_rows():
+    """A large single simulation: 2e7 coarse rows of one fine step each
+    (Larmor, 0.1 s at 5 ns): 1.3 GB of operators, 39,063 scan blocks; every
+    row against the C oracle, and the closed form."""
+    print("case: larmor 2e7 rows", flush=True)
+    psi0 = O.EIGENSTATES["half"]["plus_x"]
+    res = run_product(F.LARMOR, 0, 0.1, 5e-9, 5e-9, psi0)
+    time, u, state, spin = run_oracle(F.LARMOR, 0, 0.1, 5e-9, 5e-9, psi0)
+    n = 20000000
+    assert res.time.shape == (n,)
+    assert np.array_equal(res.time, time)
+    assert np.abs(res.time_evolution - u).max() <= TOL
+    # A chain of N = 2e7 float64 products carries ~N eps = 4e-9 of rounding;
+    # the oracle multiplies in sequence, the scan in a tree, so their states
+    # may differ by that much (both stay within it of the closed form).
+    chain = n * np.finfo(float).eps
+    t = res.time
+    exact = np.stack([np.exp(-1j * math.pi * 1000 * t), np.exp(1j * math.pi * 1000 * t)], 1)
+    exact /= math.sqrt(2)
+    e_gpu = np.abs(res.state - exact).max()
+    e_ora = np.abs(state - exact).max()
+    print(f"  vs closed form: HIP {e_gpu:.2e}, oracle {e_ora:.2e}; chain bound {chain:.2e}")
+    assert np.abs(res.state - state).max() <= chain
+    assert np.abs(res.spin - spin).max() <= chain
+    assert e_gpu <= chain and e_ora <= chain
+
+
 def test_rabi_one_full_size():
     """Spin-one Rabi drive, Lie-Trotter, 1e6 steps, plain FP64."""
     print("case: rabi_one full size", flush=True)
