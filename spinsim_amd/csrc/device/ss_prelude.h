@@ -34,9 +34,28 @@ struct ss_table {
     int zero_boundary;
 };
 
+// The interval a table was last sampled in, as the cubic
+// c0 + c1 p + c2 p^2 + c3 p^3 of p = (t - t0) inv_h; inv_h = NaN when empty.
+struct ss_interp_line {
+    double t0, inv_h, c0, c1, c2, c3;
+};
+
+#ifndef SS_INTERP_LINES
+#define SS_INTERP_LINES 4  // tables 0..3 keep their last interval per lane
+#endif
+
 struct ss_ctx {
     const ss_table* tables;
     int sim;
+    // Per-lane cache of sampled intervals: consecutive samples of one lane
+    // (a coarse step, a few knots apart at most) mostly fall in the same
+    // interval, which then costs no table reads and no divisions.
+    mutable ss_interp_line line[SS_INTERP_LINES];
+    __device__ ss_ctx() : tables(nullptr), sim(0)
+    {
+#pragma unroll
+        for (int k = 0; k < SS_INTERP_LINES; ++k) line[k].inv_h = __builtin_nan("");
+    }
 };
 
 #define SS_CONTEXT const ss_ctx* ss_ctx_ref
@@ -81,9 +100,17 @@ __device__ static inline long long ss_find_time_index(const ss_table* tb, double
     return -1;
 }
 
-// sampler(), spinsim/__init__.py:2065-2142 (orders 0, 1 and 3).
+// sampler(), spinsim/__init__.py:2065-2142 (orders 0, 1 and 3).  Linear and
+// cubic intervals are remembered per lane (ss_ctx::line) and re-evaluated as
+// polynomials in p while t stays inside them; the values agree with the
+// reference's form to rounding (~1e-16 relative).
 __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, double t)
 {
+    if (k < SS_INTERP_LINES) {
+        const ss_interp_line& ln = ctx->line[k];
+        const double p = (t - ln.t0) * ln.inv_h;
+        if (p >= 0.0 && p <= 1.0) return ((ln.c3 * p + ln.c2) * p + ln.c1) * p + ln.c0;
+    }
     const ss_table* tb = &ctx->tables[k];
     const double* time = tb->time;
     const double* amp = tb->amplitude + (long long)ctx->sim * tb->amplitude_stride;
@@ -99,7 +126,19 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
     if (tb->order == 0) {
         if (i >= 0) out = amp[i];
     } else if (tb->order == 1) {
-        if (i >= 0 && i + 1 < size) out = amp[i] * (1 - p) + amp[i + 1] * p;
+        if (i >= 0 && i + 1 < size) {
+            out = amp[i] * (1 - p) + amp[i + 1] * p;
+            const double h = time[i + 1] - time[i];
+            if (k < SS_INTERP_LINES && h > 0.0) {
+                ss_interp_line& ln = ctx->line[k];
+                ln.t0 = time[i];
+                ln.inv_h = 1.0 / h;
+                ln.c0 = amp[i];
+                ln.c1 = amp[i + 1] - amp[i];
+                ln.c2 = 0.0;
+                ln.c3 = 0.0;
+            }
+        }
     } else if (i >= 0 && i + 1 < size) {
         const double h = time[i + 1] - time[i];
         const double mid = (time[i] != time[i + 1]) ? (amp[i + 1] - amp[i]) / h : 0.0;
@@ -117,6 +156,15 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
         const double p2 = p * p, p3 = p2 * p;
         out = amp[i] * (2 * p3 - 3 * p2 + 1) + amp[i + 1] * (-2 * p3 + 3 * p2)
             + left * (p3 - 2 * p2 + p) + right * (p3 - p2);
+        if (k < SS_INTERP_LINES && h > 0.0) {
+            ss_interp_line& ln = ctx->line[k];
+            ln.t0 = time[i];
+            ln.inv_h = 1.0 / h;
+            ln.c0 = amp[i];
+            ln.c1 = left;
+            ln.c2 = 3.0 * (amp[i + 1] - amp[i]) - 2.0 * left - right;
+            ln.c3 = 2.0 * (amp[i] - amp[i + 1]) + left + right;
+        }
     }
     return out;
 }
