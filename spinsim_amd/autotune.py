@@ -132,9 +132,11 @@ def tune_lanes(sim, n_coarse: int, n_fine: int, dt: float, dt_out: float, sweep,
     u = torch.empty((n_coarse, d, d), dtype=torch.complex128, device=dev)
     sw = sweep if isinstance(sweep, torch.Tensor) else torch.as_tensor(sweep, dtype=torch.float64,
                                                                         device=dev)
+    # three untimed rounds first: the device clock ramps up over the first
+    # ~10 ms of load, which would otherwise favour the later candidates
     table = time_interleaved(
         {g: (lambda g=g: sim.launch_time_evolution(sw, t0, dt, dt_out, time_out, u, g))
-         for g in candidates}, rounds=reps)
+         for g in candidates}, rounds=reps, warmup=3)
     best = min(table, key=table.get)
     with _lock:
         _memo[key] = best
