@@ -770,7 +770,7 @@ static __device__ __forceinline__ void ss_group_reduce(ss_cd u[SS_D][SS_D], int 
 // E = U^+ U - I, in float64.  A step maps a deviation max|E| = e to about
 // 1.5 e^2, so steps repeat (at most three) until e < 1e-8, after which the
 // operator is unitary to float64 rounding and long state products keep their
-// norm.
+// norm.  An operator with e >= 1/4 is left as it is.
 static __device__ __forceinline__ void ss_reunitarise(ss_cd u[SS_D][SS_D])
 {
 #pragma unroll 1
@@ -788,6 +788,8 @@ static __device__ __forceinline__ void ss_reunitarise(ss_cd u[SS_D][SS_D])
                 e[r][c] = acc;
                 emax = fmax(emax, fmax(fabs(acc.x), fabs(acc.y)));
             }
+        // far from unitary (or NaN) the iteration need not converge: leave U
+        if (!(emax < 0.25)) break;
         ss_cd ue[SS_D][SS_D];
         ss_matmul<SS_D>(u, e, ue);
 #pragma unroll
