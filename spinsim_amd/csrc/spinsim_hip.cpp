@@ -167,6 +167,18 @@ constexpr int kSingleScanChunk = 2;   // matrices per lane
 constexpr int kSingleScanBlock = 256;
 
 template <int D>
+__device__ __forceinline__ void shfl_down_mat(const ss_cd m[D][D], ss_cd out[D][D], int s)
+{
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            out[r][c].x = __shfl_down(m[r][c].x, s, 64);
+            out[r][c].y = __shfl_down(m[r][c].y, s, 64);
+        }
+}
+
+template <int D>
 __device__ __forceinline__ void shfl_up_mat(const ss_cd m[D][D], ss_cd out[D][D], int s)
 {
 #pragma unroll
@@ -293,6 +305,65 @@ __global__ void __launch_bounds__(kSingleScanBlock) single_scan_apply(const doub
             ss_matvec<D>(m, v, t);
 #pragma unroll
             for (int r = 0; r < D; ++r) v[r] = t[r];
+        }
+    }
+}
+
+// Single simulation with at most 256 blocks, passes 2 and 3 in one: each
+// workgroup forms its own carry-in state, v_b = (T_{b-1} ... T_0) psi0, from
+// the block totals (one per thread, an ordered shuffle tree per wave, the four
+// wave products through LDS), then applies it like single_scan_apply.
+template <int D>
+__global__ void __launch_bounds__(kSingleScanBlock) single_scan_apply_carry(const double* u, long long n,
+                                                                 const double* lane_prefix,
+                                                                 const double* block_total, ss_cd p0,
+                                                                 ss_cd p1, ss_cd p2, double* state,
+                                                                 double* spin)
+{
+    __shared__ ss_cd wave_prod[kSingleScanBlock / 64][D][D];
+    __shared__ ss_cd carry[D];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    ss_cd m[D][D];
+    ss_set_identity<D>(m);
+    if (t < (int)blockIdx.x) load_mat<D>(block_total, t, m);
+    // product over the wave's lanes, later totals on the left
+#pragma unroll 1
+    for (int s = 1; s < 64; s <<= 1) {
+        ss_cd o[D][D];
+        shfl_down_mat<D>(m, o, s);
+        if ((lane & (2 * s - 1)) == 0) ss_left_multiply<D>(o, m);
+    }
+    if (lane == 0) ss_copy<D>(m, wave_prod[wave]);
+    __syncthreads();
+    if (t == 0) {
+        ss_cd pr[D][D];
+        ss_copy<D>(wave_prod[0], pr);
+        for (int w = 1; w < kSingleScanBlock / 64; ++w) ss_left_multiply<D>(wave_prod[w], pr);
+        const ss_cd psi[3] = {p0, p1, p2};
+        ss_cd v[D];
+        ss_matvec<D>(pr, psi, v);
+#pragma unroll
+        for (int r = 0; r < D; ++r) carry[r] = v[r];
+    }
+    __syncthreads();
+    const long long lane_id = (long long)blockIdx.x * kSingleScanBlock + t;
+    const long long k0 = lane_id * kSingleScanChunk;
+    if (k0 >= n) return;
+    const long long kend = k0 + kSingleScanChunk < n ? k0 + kSingleScanChunk : n;
+    ss_cd pre[D][D], c[D], v[D];
+    load_mat<D>(lane_prefix, lane_id, pre);
+#pragma unroll
+    for (int r = 0; r < D; ++r) c[r] = carry[r];
+    ss_matvec<D>(pre, c, v);
+    for (long long k = k0; k < kend; ++k) {
+        store_vec<D>(state, k, v);
+        if (spin) store_spin<D>(v, spin, k);
+        if (k + 1 < kend) {
+            ss_cd mm[D][D], tv[D];
+            load_mat<D>(u, k, mm);
+            ss_matvec<D>(mm, v, tv);
+#pragma unroll
+            for (int r = 0; r < D; ++r) v[r] = tv[r];
         }
     }
 }
@@ -1351,14 +1422,29 @@ int ss_state_scan(const double* psi0, const double* u, double* state, double* sp
     const ss_cd p0 = ss_c(psi0[0], psi0[1]), p1 = ss_c(psi0[2], psi0[3]);
     const ss_cd p2 = dim == 3 ? ss_c(psi0[4], psi0[5]) : ss_c(0, 0);
     const dim3 grid(unsigned(l.blocks)), block(kSingleScanBlock);
+    // up to 256 blocks (131,072 rows) every workgroup forms its own carry
+    // from the block totals; beyond, one workgroup scans them first
+    const bool fused = l.blocks <= kSingleScanBlock;
     if (dim == 2) {
         hipLaunchKernelGGL(single_scan_lanes<2>, grid, block, 0, s, u, (long long)n, prefix, total);
-        hipLaunchKernelGGL(single_scan_blocks<2>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
-        hipLaunchKernelGGL(single_scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
+        if (fused) {
+            hipLaunchKernelGGL(single_scan_apply_carry<2>, grid, block, 0, s, u, (long long)n, prefix, total, p0,
+                               p1, p2, state, spin);
+        } else {
+            hipLaunchKernelGGL(single_scan_blocks<2>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
+            hipLaunchKernelGGL(single_scan_apply<2>, grid, block, 0, s, u, (long long)n, prefix, carry, state,
+                               spin);
+        }
     } else {
         hipLaunchKernelGGL(single_scan_lanes<3>, grid, block, 0, s, u, (long long)n, prefix, total);
-        hipLaunchKernelGGL(single_scan_blocks<3>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
-        hipLaunchKernelGGL(single_scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state, spin);
+        if (fused) {
+            hipLaunchKernelGGL(single_scan_apply_carry<3>, grid, block, 0, s, u, (long long)n, prefix, total, p0,
+                               p1, p2, state, spin);
+        } else {
+            hipLaunchKernelGGL(single_scan_blocks<3>, dim3(1), block, 0, s, total, l.blocks, p0, p1, p2, carry);
+            hipLaunchKernelGGL(single_scan_apply<3>, grid, block, 0, s, u, (long long)n, prefix, carry, state,
+                               spin);
+        }
     }
     SS_HIP(hipGetLastError());
     return 0;
