@@ -196,11 +196,11 @@ def tune_batch(plan, reps: int = 2, use_cache: bool = True, lanes=None, items=(1
             best = tuple(map(int, disk[key]))
     table = None
     if best is None:
-        table = {}
-        for g in lanes:
-            for it in items:
-                plan.lanes, plan.items_per_grab = g, it
-                table[(g, it)] = time_launch(plan.run_time_evolution, reps=reps, warmup=warmup)
+        def run(g, it):
+            plan.lanes, plan.items_per_grab = g, it
+            plan.run_time_evolution()
+        table = time_interleaved({(g, it): (lambda g=g, it=it: run(g, it))
+                                  for g in lanes for it in items}, rounds=reps, warmup=warmup)
         best = min(table, key=table.get)
         _store_disk(key, list(best))
     plan.lanes, plan.items_per_grab = best
