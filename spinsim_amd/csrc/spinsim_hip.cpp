@@ -206,8 +206,9 @@ __device__ __forceinline__ void wave_scan(ss_cd q[D][D], ss_cd ex[D][D], int lan
 }
 
 // Workgroup-wide exclusive scan (256 lanes = 4 waves).  On return `ex` is
-// the product of the x of all earlier lanes of the workgroup and `total`
-// (valid on every lane) the product over the whole workgroup.
+// the product of the x of all earlier lanes of the workgroup; on the lanes of
+// the last wave `total` is the product over the whole workgroup (that wave's
+// own product times the earlier waves', one more matrix product).
 template <int D>
 __device__ __forceinline__ void block_scan(const ss_cd x[D][D], ss_cd ex[D][D], ss_cd total[D][D])
 {
@@ -224,8 +225,7 @@ __device__ __forceinline__ void block_scan(const ss_cd x[D][D], ss_cd ex[D][D], 
     ss_set_identity<D>(pre);
     for (int w = 0; w < wave; ++w) ss_left_multiply<D>(wave_tot[w], pre);
     ss_matmul<D>(wex, pre, ex);
-    ss_set_identity<D>(total);
-    for (int w = 0; w < kSingleScanBlock / 64; ++w) ss_left_multiply<D>(wave_tot[w], total);
+    if (wave == kSingleScanBlock / 64 - 1) ss_matmul<D>(wave_tot[wave], pre, total);
 }
 
 // Single simulation, pass 1: lane L covers states [L*C, L*C + C) and the matrices it applies,
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(kSingleScanBlock) single_scan_lanes(const doub
     ss_cd ex[D][D], total[D][D];
     block_scan<D>(x, ex, total);
     store_mat<D>(lane_prefix, lane_id, ex);
-    if (threadIdx.x == 0) store_mat<D>(block_total, blockIdx.x, total);
+    if (threadIdx.x == kSingleScanBlock - 1) store_mat<D>(block_total, blockIdx.x, total);
 }
 
 // Single simulation, pass 2 (one workgroup): carry-in state of every block,
