@@ -68,24 +68,6 @@ def _key(sim, *shape):
     return "|".join([h, name, *map(str, shape)])
 
 
-def time_launch(fn, reps=3, warmup=1):
-    """Median milliseconds of fn() on the current stream (HIP events)."""
-    import torch
-
-    for _ in range(warmup):
-        fn()
-    times = []
-    for _ in range(reps):
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        fn()
-        b.record()
-        b.synchronize()
-        times.append(a.elapsed_time(b))
-    return statistics.median(times)
-
-
 def time_interleaved(fns: dict, rounds: int = 5, warmup: int = 1):
     """Median milliseconds of each fns[key]() on the current stream, the
     candidates timed in turn within every round so that slow drifts of the
