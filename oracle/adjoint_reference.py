@@ -7,7 +7,21 @@ spinsim/__init__.py:623-686, :796-852), forms every step exponential exactly
 with scipy.linalg.expm, walks the costate backward, and takes each step's
 derivative with scipy.linalg.expm_frechet.  It shares no code with the HIP
 adjoint (which uses a series for the Frechet derivative and walks states
-backward); the rotating frequency is held fixed, as there.
+backward).
+
+The rotating frequency of coarse step i is the z field at the step's midpoint
+(halved for spin half), so a z-axis control also moves the frame: every
+frame-transformed sample of the step (x + iy turned by exp(-i m rot t_rel),
+z shifted by -m rot, m = 2 for spin half, 1 for spin one) and the frame exit
+exp(-+i rot dt_out) on the first and last components.  Per unit of the
+midpoint z field that is
+
+    sum over samples of  t_rel (dL/df_x f_y - dL/df_y f_x) - dL/df_z
+      + (dt_out / m) (Im(conj(lam_0) psi_0) - Im(conj(lam_last) psi_last))
+
+with f the frame-transformed sample and psi, lam the state and costate after
+the exit; it is scattered onto the z controls at the midpoint, so the
+gradient is that of the computed forward map.
 
 Control fields: field[axis] += scale * sampler_k(t), sampler_k an oracle
 InterpolationSampler; gradients with respect to the sampler amplitudes use
@@ -92,6 +106,9 @@ def final_state_and_gradient(static_fn, spin, controls, t0, t1, dt, dt_out, psi0
         units.append([O.InterpolationSampler(time, np.eye(len(time))[m], order, zb)
                       for m in range(len(time))])
     exit_at = {k: x for k, x in exits}
+    exit_row = {k: r for r, (k, _) in enumerate(exits)}
+    m_frame = 2.0 if spin == O.HALF else 1.0
+    dz_mid = np.zeros(n)  # dL/d(z field at the midpoint of coarse step i)
     lie = 3 if spin == O.HALF else 4
     ops = JH if spin == O.HALF else J1
     w0 = (1.5 + O.SQRT3) / 6
@@ -99,6 +116,9 @@ def final_state_and_gradient(static_fn, spin, controls, t0, t1, dt, dt_out, psi0
     dg = {}
     for idx in range(len(steps) - 1, -1, -1):
         if idx + 1 in exit_at:
+            psi_x = states[idx + 1] if idx + 1 < len(states) else psi_t
+            dz_mid[exit_row[idx + 1]] += dt_out / m_frame * (
+                np.imag(np.conj(lam[0]) * psi_x[0]) - np.imag(np.conj(lam[-1]) * psi_x[-1]))
             lam = np.conj(exit_at[idx + 1]) * lam
         i, j, which, a, g = steps[idx]
         psi_b = states[idx]
@@ -113,9 +133,11 @@ def final_state_and_gradient(static_fn, spin, controls, t0, t1, dt, dt_out, psi0
             d1, d2 = dg[(i, j, 0)], dg[(i, j, 1)]
             dfa = dt2 * (w0 * d1 + w1 * d2)
             dfb = dt2 * (w1 * d1 + w0 * d2)
-            for df, tkey, wkey in ((dfa, "ta", "wa"), (dfb, "tb", "wb")):
+            for df, tkey, wkey, s in ((dfa, "ta", "wa", "a"), (dfb, "tb", "wb", "b")):
                 lab = df.copy()
                 if rotating:
+                    f = rec["f" + s][:, i]
+                    dz_mid[i] += rec["r" + s][i] * (df[0] * f[1] - df[1] * f[0]) - df[2]
                     w = rec[wkey][i]
                     wf = w * w if spin == O.HALF else w
                     gw = (df[0] + 1j * df[1]) * wf
@@ -127,4 +149,10 @@ def final_state_and_gradient(static_fn, spin, controls, t0, t1, dt, dt_out, psi0
                         wgt = us(t)
                         if wgt != 0:
                             grads[c][m] += v * wgt
+        if rotating:
+            t_mid = record[0]["tc"][i] + dt_out / 2
+            for c, (axis, time, amp, order, zb, scale) in enumerate(controls):
+                if axis == 2:
+                    for m, us in enumerate(units[c]):
+                        grads[c][m] += scale * dz_mid[i] * us(np.atleast_1d(t_mid))[0]
     return psi_t, grads

@@ -347,7 +347,8 @@ def get_time_evolution(get_field, spin, time_start, time_end, time_step_integrat
     NumPy ufuncs follows the reference's per-sample callback (:620) lane by
     lane.  ``coarse_index`` restricts the run to a subset of coarse steps (the
     result rows then follow that subset).  ``record``, if a list, receives one
-    dict per fine step (CF4 only): the lab sample times, windings, and the
+    dict per fine step (CF4 only): the lab sample times and their offsets from
+    the coarse time, windings, the frame-transformed field samples, and the
     generators g of its two exponentials, lanes along the last axis.
     """
     dim, lie, exponentiation, integration = resolve_methods(spin, exponentiation, integration)
@@ -395,8 +396,10 @@ def get_time_evolution(get_field, spin, time_start, time_end, time_step_integrat
             g2 = [dt * (w1 * fa[k] + w0 * fb[k]) for k in range(lie)]
             u = append(g2)
             if record is not None:
-                record.append({"ta": (time_fine + 0.5 * dt * (1 - 1 / SQRT3) - time_coarse)
-                               + time_coarse, "tb": t_rel + time_coarse, "wa": wa, "wb": wb,
+                ra = (time_fine + 0.5 * dt * (1 - 1 / SQRT3)) - time_coarse
+                record.append({"ta": ra + time_coarse, "tb": t_rel + time_coarse, "wa": wa,
+                               "wb": wb, "ra": ra, "rb": t_rel, "fa": fa.copy(),
+                               "fb": fb.copy(), "tc": time_coarse,
                                "g1": np.array(g), "g2": np.array(g2), "dt": dt,
                                "rot": rotating_wave})
         elif integration == HEUN:                                  # :692-752

@@ -15,8 +15,10 @@ time-evolution launch and the per-simulation state scan and keeps only the
 coarse step operators and states (checkpoints); the backward pass scans the
 costates backward over the coarse operators, then recomputes every fine step
 and accumulates dL/dc with the Frechet derivative of each step exponential
-(``ss_batch_adjoint``).  The rotating frequency of each coarse step is held
-fixed in the derivative (the exact evolution does not depend on the frame).
+(``ss_batch_adjoint``).  In the rotating frame a z-axis control also moves
+each coarse step's rotating frequency (the z field at the step's midpoint,
+spinsim/__init__.py:823-829); that term is included, so the gradient is the
+derivative of the computed forward map for every integration method.
 """
 from __future__ import annotations
 

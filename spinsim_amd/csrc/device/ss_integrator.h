@@ -959,6 +959,17 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
 // Within a coarse interval the state and costate are both walked backward
 // from the interval's end (the steps are unitary), so only the coarse
 // operators and states are stored; the fine steps are recomputed.
+//
+// In the rotating frame the frequency of a coarse step is the z field at its
+// midpoint (halved for spin half), so a z-axis control also moves the frame:
+// each transformed sample (x + iy turned by exp(-i m rot t_rel), z shifted by
+// -m rot; m = 2 for spin half, 1 for spin one) and the frame exit, which
+// turns the first and last components by exp(-+i rot dt_out).  Per unit of
+// the midpoint z field,
+//     sum_samples t_rel (dL/df_x f_y - dL/df_y f_x) - dL/df_z
+//       + (dt_out / m) (Im(conj(lam_0) psi_0) - Im(conj(lam_last) psi_last)),
+// f the transformed sample, psi and lam at the interval's end; the sum is
+// scattered onto the z controls at the midpoint.
 // ===========================================================================
 #define SS_ADJ_ORDER 6       // Frechet series order (terms up to n = 6)
 #define SS_ADJ_TAYLOR 9      // exp(-A) v Taylor order
@@ -1239,14 +1250,22 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
     double* grad_row = a.grad + (long long)sim * a.grad_stride;
 
     unsigned mask = 0;
+    bool z_control = false;
     long long base[SS_MAX_CONTROLS];
     for (int c = 0; c < a.n_controls; ++c) {
         const int ax = a.controls[c].axis;
         mask |= (ax <= 1 && SS_ROT) ? 3u : (1u << ax);
+        z_control = z_control || ax == 2;
         double p;
         const long long i0 = ss_find_time_index(&ctx.tables[a.controls[c].table], tc, &p);
         base[c] = (i0 < 0 ? 0 : i0) - 1;
     }
+#if SS_ROT
+    // the frame term of a z control needs dL/dg of x, y and z
+    if (z_control) mask |= 7u;
+#else
+    (void)z_control;
+#endif
 
     // state and costate at the interval's end, in the rotating frame
     ss_cd u[SS_D][SS_D], psi0[SS_D], psi[SS_D], lam[SS_D];
@@ -1276,6 +1295,13 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
         psi[SS_D - 1] = ss_mul(psi[SS_D - 1], ss_conj(w));
         lam[SS_D - 1] = ss_mul(lam[SS_D - 1], ss_conj(w));
     }
+    // dL/d(z field at the interval's midpoint) through the rotating frequency;
+    // the frame exit's part (a common phase per component leaves it unchanged)
+    double dz_mid = 0.0;
+    if (z_control)
+        dz_mid = dt_out / (SS_DIM == 2 ? 2.0 : 1.0) *
+                 ((lam[0].x * psi[0].y - lam[0].y * psi[0].x) -
+                  (lam[SS_D - 1].x * psi[SS_D - 1].y - lam[SS_D - 1].y * psi[SS_D - 1].x));
 #endif
     const double half = 0.5 * dt;
     const double off_a = half * a.off_a_frac, off_b = half * a.off_b_frac;
@@ -1297,6 +1323,9 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
         ss_adjoint_exp(g1, psi, lam, mask, d1, a.status);
 #pragma unroll
         for (int k = 0; k < SS_LIE; ++k) dfa[k] = dt * d1[k];
+#if SS_ROT
+        if (z_control) dz_mid += tra * (dfa[0] * fa[1] - dfa[1] * fa[0]) - dfa[2];
+#endif
         ss_unframe(dfa, wa, lab);
         ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
 #else
@@ -1334,12 +1363,23 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
             dfb[k] = 0.5 * dt * d2[k];
         }
 #endif
+#if SS_ROT
+        if (z_control)
+            dz_mid += tra * (dfa[0] * fa[1] - dfa[1] * fa[0]) - dfa[2] +
+                      trb * (dfb[0] * fb[1] - dfb[1] * fb[0]) - dfb[2];
+#endif
         ss_unframe(dfa, wa, lab);
         ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
         ss_unframe(dfb, wb, lab);
         ss_scatter_sample(a, &ctx, trb + tc, lab, win, base, grad_row);
 #endif
     }
+#if SS_ROT
+    if (z_control) {
+        const double dmid[4] = {0.0, 0.0, dz_mid, 0.0};
+        ss_scatter_sample(a, &ctx, tc + dt_out / 2, dmid, win, base, grad_row);
+    }
+#endif
     for (int c = 0; c < a.n_controls; ++c) {
         const long long size = ctx.tables[a.controls[c].table].size;
 #pragma unroll
