@@ -17,7 +17,11 @@ uneven durations (1-50 ms, 50 ns steps, 1 us outputs; float32-bulk; one
 persistent launch + per-simulation state scans replayed as a HIP graph), each
 with its state error against the float64 oracle; and the differentiation
 workload (1024 spin-one pulses, forward + adjoint gradient, float64 and
-float32-bulk forward), with its gradient error against central differences.
+float32-bulk forward), with its gradient error against central differences;
+the long noisy spin-half run (1e7 steps); and the spin-half Larmor precession
+(1e6 steps, analytic exponentials).  The batched, long noisy and Larmor
+variants also time the oracle's C restatement on the host cores over the
+oracle runs they check against (``cpu_steps_per_s``, with the sample named).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -33,7 +37,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-VARIANTS = ("fp32_bulk", "batched", "adjoint", "long_noisy")
+VARIANTS = ("fp32_bulk", "batched", "adjoint", "long_noisy", "larmor")
 METRIC = "Magnus-CF4 steps/sec on 1 GPU at state err ≤1e-6 vs FP64 oracle; batched/plain"
 
 RABI_ONE = """
@@ -41,6 +45,11 @@ __device__ void get_field(double t, const double* s, double* f)
 { f[0] = SS_TAU * 2e3 * cos(SS_TAU * 7e5 * t); f[1] = 0; f[2] = SS_TAU * 7e5; f[3] = SS_TAU * 1e3; }
 """
 T0, T1, DT, DT_OUT = 0.0, 100e-3, 100e-9, 1e-6
+
+LARMOR = """
+__device__ void get_field(double t, const double* s, double* f)
+{ f[0] = 0; f[1] = 0; f[2] = SS_TAU * 1e3; }
+"""
 
 # Swept spin-half Rabi drive: sweep = (carrier, Rabi amplitude, detuning), rad/s.
 RABI_SWEEP = """
@@ -103,31 +112,32 @@ def evaluate_end_to_end(ss, torch, lanes, reps=3):
     return statistics.median(times)
 
 
-def single_workload(ss, torch, precision, steps, warmup, lanes):
-    """Time K evaluate() passes of the spin-one workload; returns a dict and the state."""
+def resident_passes(ss, torch, sim, psi0, t0, t1, dt_in, dt_out, steps, warmup, lanes):
+    """Time K evaluate() passes of one simulation over device-resident buffers
+    (time evolution, then the state scan with the Bloch vectors fused);
+    returns a dict, the states and the time stamps."""
     from spinsim_amd.simulator import derive_steps
 
     dev = torch.device("cuda", 0)
-    sim = ss.Simulator(RABI_ONE, ss.SpinQuantumNumber.ONE, precision=precision)
-    dt, n, _ = derive_steps(T0, T1, DT, DT_OUT)
-    nf = math.floor(DT_OUT / dt + 0.5)
-    psi0 = ss.SpinQuantumNumber.ONE.plus_z
+    d = sim.dimension
+    dt, n, _ = derive_steps(t0, t1, dt_in, dt_out)
+    nf = math.floor(dt_out / dt + 0.5)
     sweep = torch.zeros(1, dtype=torch.float64, device=dev)
     time_out = torch.empty(n, dtype=torch.float64, device=dev)
-    u = torch.empty((n, 3, 3), dtype=torch.complex128, device=dev)
-    state = torch.empty((n, 3), dtype=torch.complex128, device=dev)
+    u = torch.empty((n, d, d), dtype=torch.complex128, device=dev)
+    state = torch.empty((n, d), dtype=torch.complex128, device=dev)
     spin = torch.empty((n, 3), dtype=torch.float64, device=dev)
     ws = sim.scan_workspace(n, dev)
     stream = torch.cuda.current_stream(dev)
     table = None
     if lanes == "auto":
         from spinsim_amd.autotune import tune_lanes
-        lanes, table = tune_lanes(sim, n, nf, dt, DT_OUT, sweep, T0)
+        lanes, table = tune_lanes(sim, n, nf, dt, dt_out, sweep, t0)
 
     def step(events=None):
         if events is not None:
             events[0].record(stream)
-        sim.launch_time_evolution(sweep, T0, dt, DT_OUT, time_out, u, lanes)
+        sim.launch_time_evolution(sweep, t0, dt, dt_out, time_out, u, lanes)
         if events is not None:
             events[1].record(stream)
         sim.launch_state_scan(psi0, u, state, ws, spin=spin)
@@ -147,11 +157,59 @@ def single_workload(ss, torch, precision, steps, warmup, lanes):
            "kernel_ms": kernel_ms, "lanes_per_step": lanes, "coarse": n, "fine": n * nf,
            "autotune_ms": None if table is None else {str(k): v for k, v in table.items()},
            "dt": dt, "n_fine": nf}
-    return out, state.cpu().numpy(), psi0
+    return out, state.cpu().numpy(), time_out.cpu().numpy()
 
 
-def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
-    """4096 spin-half Rabi simulations with uneven durations in one persistent launch."""
+def single_workload(ss, torch, precision, steps, warmup, lanes):
+    """Time K evaluate() passes of the spin-one workload; returns a dict and the state."""
+    sim = ss.Simulator(RABI_ONE, ss.SpinQuantumNumber.ONE, precision=precision)
+    psi0 = ss.SpinQuantumNumber.ONE.plus_z
+    out, state, _ = resident_passes(ss, torch, sim, psi0, T0, T1, DT, DT_OUT, steps, warmup,
+                                    lanes)
+    return out, state, psi0
+
+
+def larmor_workload(ss, torch, steps, warmup, cpu_cores):
+    """Spin-half Larmor precession, field (0, 0, 2pi 1 kHz), t = 0 - 100 ms, dt 100 ns,
+    output 500 ns (2e5 x 5 = 1e6 CF4 steps), +x, analytic exponentials, rotating
+    frame, float64: steps/s per device-resident evaluate pass, state error against
+    the closed form psi(t) = (exp(-i pi 1e3 t), exp(i pi 1e3 t)) / sqrt 2 and
+    against the oracle, and the oracle's C restatement timed on the host cores."""
+    import numpy as np
+
+    from oracle.c_oracle import COracle
+
+    sim = ss.Simulator(LARMOR, ss.SpinQuantumNumber.HALF,
+                       exponentiation_method=ss.ExponentiationMethod.ANALYTIC)
+    psi0 = ss.SpinQuantumNumber.HALF.plus_x
+    out, state, t = resident_passes(ss, torch, sim, psi0, 0.0, 100e-3, 100e-9, 500e-9, steps,
+                                    warmup, "auto")
+    exact = np.stack([np.exp(-1j * math.pi * 1e3 * t), np.exp(1j * math.pi * 1e3 * t)], 1)
+    exact /= math.sqrt(2)
+    co = COracle(LARMOR, 2, expo="analytic")
+    n, nf, dt = out["coarse"], out["n_fine"], out["dt"]
+    rates, ref = [], None
+    for _ in range(3 if cpu_cores else 1):
+        c0 = time.perf_counter()
+        _, u = co.time_evolution([0.0], 0.0, dt, 500e-9, nf, n, threads=cpu_cores or None)
+        ref = co.get_state(psi0, u)
+        co.get_spin(ref, threads=cpu_cores or None)
+        rates.append(n * nf / (time.perf_counter() - c0))
+    res = {k: out[k] for k in ("steps_per_s", "ms_per_step", "kernel_ms", "lanes_per_step",
+                               "coarse", "fine")}
+    res["state_err_vs_closed_form"] = float(np.abs(state - exact).max())
+    res["state_err_vs_oracle"] = float(np.abs(state - ref).max())
+    if cpu_cores:
+        res["cpu_steps_per_s"] = statistics.median(rates)
+        res["cpu_sample"] = (f"whole workload (time evolution, state product, spin) on "
+                             f"{cpu_cores} host threads, median of 3")
+    return res
+
+
+def batched_workload(ss, torch, steps, warmup, cpu_cores=None, n_sims=4096, check=8):
+    """4096 spin-half Rabi simulations with uneven durations in one persistent launch;
+    the oracle runs of the sampled simulations (float64, C restatement on
+    cpu_cores host threads) give the CPU rate."""
     import numpy as np
 
     from oracle import spinsim_oracle as O
@@ -178,13 +236,22 @@ def batched_workload(ss, torch, steps, warmup, n_sims=4096, check=8):
     elapsed = time.perf_counter() - t_start
     errs = []
     co = COracle(RABI_SWEEP, 2)
+    cpu_steps, cpu_s = 0, 0.0
     for s in np.linspace(0, n_sims - 1, check).astype(int):
         dt, n, nf, _ = O.timing(0.0, dur[s], 50e-9, 1e-6)
-        _, u = co.time_evolution(sweep[s], 0.0, dt, 1e-6, nf, n)
+        c0 = time.perf_counter()
+        _, u = co.time_evolution(sweep[s], 0.0, dt, 1e-6, nf, n, threads=cpu_cores)
         ref = co.get_state(psi0, u)
+        cpu_s += time.perf_counter() - c0
+        cpu_steps += n * nf
         a, b = int(plan.offsets_host[s]), int(plan.offsets_host[s + 1])
         errs.append(float(np.abs(plan.state[a:b].cpu().numpy() - ref).max()))
-    return {"steps_per_s": plan.fine_steps / (elapsed / steps), "ms_per_step": 1e3 * elapsed / steps,
+    cpu = {}
+    if cpu_cores:
+        cpu = {"cpu_steps_per_s": cpu_steps / cpu_s,
+               "cpu_sample": f"the {check} sampled simulations ({cpu_steps} CF4 steps, float64 "
+                             f"time evolution and state product) on {cpu_cores} host threads"}
+    return {**cpu, "steps_per_s": plan.fine_steps / (elapsed / steps), "ms_per_step": 1e3 * elapsed / steps,
             "fine_steps": plan.fine_steps, "coarse_rows": plan.total, "simulations": n_sims,
             "lanes_per_step": plan.lanes, "items_per_grab": plan.items_per_grab,
             "scan_chunk_rows": plan.scan_chunk_rows,
@@ -225,11 +292,14 @@ def make_long_noisy_field(noise_sampler):
     return get_field
 
 
-def long_noisy_workload(ss, torch, reps=3):
+def long_noisy_workload(ss, torch, reps=3, cpu_cores=None):
     """1e7 CF4 steps (dt 10 ns, output 1 us), float64 and float32-bulk: steps/s,
-    unitarity drift max |U^+ U - I| of the coarse operators, and the float32-bulk
-    state error against float64."""
+    unitarity drift max |U^+ U - I| of the coarse operators, the float32-bulk
+    state error against float64, both state errors against the oracle, and the
+    oracle's C restatement timed on cpu_cores host threads."""
     import numpy as np
+
+    from oracle.c_oracle import COracle
 
     sweep, grid, noise = long_noisy_inputs()
     sampler = ss.generate_interpolation_sampler(grid, noise)
@@ -255,6 +325,19 @@ def long_noisy_workload(ss, torch, reps=3):
                      "unitarity_drift": drift}
     out["fp32_bulk"]["state_err_vs_fp64"] = float(np.abs(states["fp32_bulk"] -
                                                          states["fp64"]).max())
+    src, _, _ = ss.transpile(field)
+    co = COracle(src, 2, sampler_names=("noise_table",))
+    co.set_samplers([(grid, noise, 3, True)])
+    c0 = time.perf_counter()
+    _, u = co.time_evolution(sweep, 0.0, 1e-8, 1e-6, 100, 100000, threads=cpu_cores)
+    ref = co.get_state(psi0, u)
+    cpu_s = time.perf_counter() - c0
+    for prec in ("fp64", "fp32_bulk"):
+        out[prec]["state_err_vs_oracle"] = float(np.abs(states[prec] - ref).max())
+    if cpu_cores:
+        out["cpu_steps_per_s"] = 1e7 / cpu_s
+        out["cpu_sample"] = (f"whole workload (1e7 CF4 steps, float64 time evolution and state "
+                             f"product) on {cpu_cores} host threads, one run")
     return out
 
 
@@ -262,11 +345,24 @@ def _z_bias(time_sample, sweep_parameters, field_sample):
     field_sample[2] = 2 * math.pi * 1e5
 
 
-def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_coords=8):
+# The differentiation workload's field for the oracle: two per-simulation
+# cubic control tables on x and y and the z bias.
+CONTROLLED_ONE = """
+__device__ void get_field(double t, const double* s, double* f, SS_CONTEXT)
+{ f[0] = ctl_x(t); f[1] = ctl_y(t); f[2] = SS_TAU * 1e5; f[3] = 0; }
+"""
+
+
+def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_coords=8,
+                     cpu_cores=None, check=8):
     """1024 spin-one state-transfer pulses (2 ms, 20 ns steps, 2000 cubic control
     samples on x and y), loss 1 - |psi(T)[-1]|^2: forward + adjoint gradient,
-    checked against central differences of the float64 forward."""
+    checked against central differences of the float64 forward; psi(T) of
+    `check` sampled pulses against the oracle, whose float64 forward runs (C
+    restatement on cpu_cores host threads) give the CPU rate."""
     import numpy as np
+
+    from oracle.c_oracle import COracle
 
     rng = np.random.default_rng(20261202)
     t_end, m, sigma = 2e-3, 2000, 2 * math.pi * 5e3
@@ -282,15 +378,26 @@ def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_co
         a = amps.clone().requires_grad_(True)
         psi = cs.final_state(a, 0.0, t_end, 20e-9, 1e-6, psi0)
         (1 - psi[:, -1].abs() ** 2).sum().backward()
-        return a.grad
+        return a.grad, psi
 
-    grad = run()
+    grad, psi_t = run()
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(reps):
-        grad = run()
+        grad, psi_t = run()
     torch.cuda.synchronize()
     elapsed = (time.perf_counter() - t) / reps
+    psi_t = psi_t.detach().cpu().numpy()
+    co = COracle(CONTROLLED_ONE, 3, sampler_names=("ctl_x", "ctl_y"))
+    co.set_samplers([(tk, amp[:, 0], 3, True), (tk, amp[:, 1], 3, True)])
+    psi_err, cpu_s = 0.0, 0.0
+    for s_ in np.linspace(0, n_sims - 1, check).astype(int):
+        c0 = time.perf_counter()
+        _, u = co.time_evolution([0.0], 0.0, 20e-9, 1e-6, 50, 2000, sim=int(s_),
+                                 threads=cpu_cores)
+        st = co.get_state(psi0, u)
+        cpu_s += time.perf_counter() - c0
+        psi_err = max(psi_err, float(np.abs(u[-1] @ st[-1] - psi_t[s_]).max()))
     steps = n_sims * 100000
     # central differences (float64 forward), all perturbations in one batch
     ref = cs if precision == "fp64" else ss.ControlledSimulator(
@@ -313,10 +420,17 @@ def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_co
     g = grad.cpu().numpy()
     ad = np.array([g[s_, k, j] for (s_, k, j) in coords])
     rel = float(np.abs(fd - ad).max() / np.abs(fd).max())
+    cpu = {}
+    if cpu_cores:
+        cpu = {"cpu_forward_steps_per_s": check * 100000 / cpu_s,
+               "cpu_sample": f"float64 forward (time evolution and state product) of the {check} "
+                             f"sampled pulses ({check * 100000} CF4 steps) on {cpu_cores} host "
+                             "threads; the oracle has no adjoint"}
     return {"forward_backward_ms": elapsed * 1e3, "steps_per_s": 2 * steps / elapsed,
             "fine_steps_per_pass": steps, "simulations": n_sims, "control_samples": 2 * m,
             "grad_rel_err_vs_central_fd": rel, "fd_coordinates": len(coords),
-            "precision": precision}
+            "final_state_err_vs_oracle_sampled": psi_err, "sampled_simulations": check,
+            "precision": precision, **cpu}
 
 
 def fma_probe(torch, threads=256 * 4 * 8 * 64, iters=8192, reps=3):
@@ -379,6 +493,8 @@ def main():
     e2e_ms = evaluate_end_to_end(ss, torch, main_run["lanes_per_step"]) \
         if args.precision == "fp64" else None
 
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    cpu_cores = None if args.no_cpu_baseline else cores
     variants = {}
     if "fp32_bulk" in chosen:
         v32, state32, _ = single_workload(ss, torch, "fp32_bulk", args.steps, args.warmup, lanes)
@@ -386,12 +502,17 @@ def main():
                                                       "lanes_per_step")}
     if "batched" in chosen:
         variants["batched_4096_spin_half"] = batched_workload(ss, torch, max(3, args.steps // 4),
-                                                              1)
+                                                              1, cpu_cores)
     if "adjoint" in chosen:
         for prec in ("fp64", "fp32_bulk"):
-            variants[f"adjoint_1024_spin_one_{prec}"] = adjoint_workload(ss, torch, prec)
+            variants[f"adjoint_1024_spin_one_{prec}"] = adjoint_workload(
+                ss, torch, prec, cpu_cores=cpu_cores if prec == "fp64" else None)
     if "long_noisy" in chosen:
-        variants["long_noisy_1e7_spin_half"] = long_noisy_workload(ss, torch)
+        variants["long_noisy_1e7_spin_half"] = long_noisy_workload(ss, torch,
+                                                                   cpu_cores=cpu_cores)
+    if "larmor" in chosen:
+        variants["larmor_1e6_spin_half"] = larmor_workload(ss, torch, args.steps, args.warmup,
+                                                           cpu_cores)
 
     if args.copy_probe:
         src = torch.empty(args.copy_probe << 17, dtype=torch.float64, device="cuda")
@@ -401,7 +522,6 @@ def main():
 
     # Accuracy against the float64 oracle (C restatement) on the same inputs.
     co = COracle(RABI_ONE, 3)
-    cores = min(len(os.sched_getaffinity(0)), 16)
     base = None
     if args.no_cpu_baseline:
         _, u_ref = co.time_evolution([0.0], T0, dt, DT_OUT, nf, n, threads=cores)
