@@ -63,6 +63,10 @@ __device__ void get_field(double t, const double* s, double* f)
 # squarings x 207 + 143 set-up) + 2 matrix products x 198 + frame 24 + CF4
 # combinations 32 + sampling 14 + time update 1.
 FLOPS_PER_STEP = 2 * (24 * 207 + 143) + 2 * 198 + 24 + 32 + 14 + 1
+# What the kernel executes: its residual square forms two rows of the product
+# and derives the third from special unitarity, 185 flops (84 FMA, 12 MUL,
+# 5 ADD) where the reference's full square takes 207.
+FLOPS_PER_STEP_EXECUTED = 2 * (24 * 185 + 143) + 2 * 198 + 24 + 32 + 14 + 1
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per time-evolution launch of the headline workload (phased launch
 # on one and on three waves per SIMD read the same), from rocprofv3 FETCH_SIZE /
@@ -489,6 +493,7 @@ def main():
                                             lanes)
     n, nf, dt = main_run["coarse"], main_run["n_fine"], main_run["dt"]
     achieved = main_run["fine"] * FLOPS_PER_STEP / (main_run["kernel_ms"] * 1e-3) / 1e12
+    executed = main_run["fine"] * FLOPS_PER_STEP_EXECUTED / (main_run["kernel_ms"] * 1e-3) / 1e12
     probe = fma_probe(torch)
     e2e_ms = evaluate_end_to_end(ss, torch, main_run["lanes_per_step"]) \
         if args.precision == "fp64" else None
@@ -558,8 +563,10 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC_BYTES},
+        "fp64_executed": {"flops_per_step": FLOPS_PER_STEP_EXECUTED,
+                          "tflops": executed, "frac_of_peak": executed / FP64_PEAK_TFLOPS},
         "fp64_fma_probe": {"tflops": probe, "frac_of_published_peak": probe / FP64_PEAK_TFLOPS,
-                           "kernel_frac_of_probe": achieved / probe},
+                           "kernel_frac_of_probe": executed / probe},
         "evaluate_numpy_ms": e2e_ms,
         "cpu_baseline": base,
         "cpu_baseline_1core": base1,

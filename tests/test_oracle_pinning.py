@@ -108,6 +108,69 @@ def test_lie_trotter_squares_converge():
         assert b < a / 10  # each hypercube divides the splitting error by 16
 
 
+def _square_derived_row(m):
+    """(1 + m)^2 - 1 with the last row taken from special unitarity
+    (spin half a10 = -conj a01, a11 = conj a00; spin one a2 = conj((-a02, -a12,
+    a00 + a11) + a0 x a1)), the form the HIP squaring uses."""
+    d = len(m)
+    b = [[m[r][c] + (2 if r == c else 0) for c in range(d)] for r in range(d)]
+    t = [[sum(m[r][k] * b[k][c] for k in range(d)) for c in range(d)] for r in range(d - 1)]
+    if d == 2:
+        return [t[0], [-np.conj(t[0][1]), np.conj(t[0][0])]]
+    a0, a1 = t
+    return [a0, a1, [np.conj(-a0[2] + a0[1] * a1[2] - a0[2] * a1[1]),
+                     np.conj(-a1[2] + a0[2] * a1[0] - a0[0] * a1[2]),
+                     np.conj(a0[0] + a1[1] + a0[0] * a1[1] - a0[1] * a1[0])]]
+
+
+@pytest.mark.parametrize("fn,lie,q_zero,squares,bound", [
+    ("matrix_exponential_lie_trotter_half", 3, False, 1, 1e-14),
+    ("matrix_exponential_lie_trotter_half", 3, False, 7, 1e-14),
+    ("matrix_exponential_lie_trotter_eight", 8, False, 1, 1e-14),
+    ("matrix_exponential_lie_trotter_eight", 8, False, 7, 1e-14),
+    ("matrix_exponential_lie_trotter_one", 4, True, 1, 1e-14),
+    ("matrix_exponential_lie_trotter_one", 4, False, 24, 1e-14),
+])
+def test_lie_trotter_starts_special_unitary(monkeypatch, fn, lie, q_zero, squares, bound):
+    """The Lie-Trotter starts are special unitary, so squaring with the last row
+    derived from the others equals the full square to rounding: spin half and
+    the eight generators at any number of squares, spin one when q = 0, and
+    spin one at 24 squares for |g| <= 1."""
+    rng = np.random.default_rng(11)
+    g = [rng.normal(0, 0.5, 200) for _ in range(lie)]
+    if q_zero:
+        g[3] = np.zeros(200)
+    full = np.array(getattr(O, fn)(g, squares))
+    monkeypatch.setattr(O, "matrix_square_m1", _square_derived_row)
+    derived = np.array(getattr(O, fn)(g, squares))
+    assert np.abs(full - derived).max() < bound
+
+
+def test_spin_one_lie_trotter_start_quirk():
+    """The reference's spin-one start (:1780-1792) puts exp(2i(q - p)) on entry
+    (0, 2), where the symmetric product exp(-iD/2) exp(-iR) exp(-iD/2) has
+    exp(-2i(q + p)); with q != 0 the start is then not special unitary, and at
+    few squares the full square and the derived-row square differ visibly."""
+    g = [np.array([0.3]), np.array([-0.2]), np.array([0.4]), np.array([0.5])]
+    start = np.array(O.matrix_exponential_lie_trotter_one(g, 0))[:, :, 0]
+    a = math.hypot(0.3, -0.2)
+    p, q = math.atan2(-0.2, 0.3), 0.5 / 6
+    assert abs(start[0, 2] - (-math.sin(a / 2) ** 2 * np.exp(2j * (q - p)))) < 1e-15
+    d = np.diag(np.exp(-1j * np.array([0.2 + q, -2 * q, -0.2 + q])))
+    rot = expm(-1j * (0.3 * hamiltonian([1, 0, 0, 0], "one") - 0.2 * hamiltonian([0, 1, 0, 0], "one")))
+    strang = d @ rot @ d
+    off = np.abs(start - strang)
+    assert off[0, 2] > 1e-3 and np.delete(off.ravel(), 2).max() < 1e-14
+    full = np.array(O.matrix_exponential_lie_trotter_one([v * 16 for v in g], 2))
+    saved = O.matrix_square_m1
+    try:
+        O.matrix_square_m1 = _square_derived_row
+        derived = np.array(O.matrix_exponential_lie_trotter_one([v * 16 for v in g], 2))
+    finally:
+        O.matrix_square_m1 = saved
+    assert np.abs(full - derived).max() > 1e-6
+
+
 @pytest.mark.parametrize("spin,expo", [("half", "analytic"), ("half", "lie_trotter"),
                                        ("one", "lie_trotter")])
 def test_c_oracle_single_step_is_expm(spin, expo):
