@@ -63,9 +63,11 @@ __device__ void get_field(double t, const double* s, double* f)
 # squarings x 207 + 143 set-up) + 2 matrix products x 198 + frame 24 + CF4
 # combinations 32 + sampling 14 + time update 1.
 FLOPS_PER_STEP = 2 * (24 * 207 + 143) + 2 * 198 + 24 + 32 + 14 + 1
-# What the kernel executes: its residual square forms two rows of the product
-# and derives the third from special unitarity, 185 flops (84 FMA, 12 MUL,
-# 5 ADD) where the reference's full square takes 207.
+# The same count for the kernel's method: its residual square forms two rows
+# of the product and derives the third from special unitarity, 185 flops (84
+# FMA, 12 MUL, 5 ADD) where the reference's full square counts 207 (and
+# compiles to 201).  rocprofv3's counters read 9,929 per step for the
+# three-wave launch, which also does the two-lane row products.
 FLOPS_PER_STEP_EXECUTED = 2 * (24 * 185 + 143) + 2 * 198 + 24 + 32 + 14 + 1
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per time-evolution launch of the headline workload (phased launch
