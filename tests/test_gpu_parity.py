@@ -229,6 +229,26 @@ def test_number_of_squares(squares):
     assert_close(res, ref)
 
 
+STRONG_ONE = F.Field("strong_one", """
+__device__ void get_field(double t, const double* s, double* f)
+{ f[0] = SS_TAU * 1.2e5 * cos(SS_TAU * 3e4 * t); f[1] = SS_TAU * 9e4 * sin(SS_TAU * 5e4 * t);
+  f[2] = SS_TAU * 1.5e5 * cos(SS_TAU * 2e4 * t); f[3] = SS_TAU * 2e5; }
+""", None, "one")
+
+
+@pytest.mark.parametrize("rotating", [True, False])
+def test_strong_spin_one_steps(rotating):
+    """Coarse 1 us steps of a strong spin-one field with a quadratic shift, so
+    |g| per exponential reaches ~1-2: the regime where the reference's
+    Lie-Trotter start departs most from special unitarity and the kernel's
+    derived-row squares differ most from full squares (a few 1e-15 per
+    exponential at 24 squares), held to 1e-9 over 4,000 steps."""
+    print(f"case: strong spin-one steps rotating={rotating}", flush=True)
+    psi0 = O.EIGENSTATES["one"]["plus_x"]
+    args = (STRONG_ONE, 0, 4e-3, 1e-6, 1e-5, psi0)
+    assert_close(run_product(*args, rotating=rotating), run_oracle(*args, rotating=rotating))
+
+
 def test_edge_sizes():
     print("case: edge sizes", flush=True)
     psi0 = O.EIGENSTATES["half"]["plus_y"]
