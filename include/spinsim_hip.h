@@ -162,8 +162,9 @@ size_t ss_state_scan_workspace(int64_t n, int dim);
  * autotuner.) */
 size_t ss_batch_scan_workspace(int64_t total, int dim, int64_t chunk_rows);
 /* state[0] = psi0, state[i] = u[i-1] state[i-1] for i < n (u[n-1] unused),
- * as a prefix product in float64 (three passes: lane and workgroup products,
- * workgroup carries, states).  psi0: HOST array of dim complex values; u,
+ * as a prefix product in float64 (two passes up to 131,072 rows: lane and
+ * workgroup products, then each workgroup's carry and its states; beyond,
+ * the carries get a pass of their own).  psi0: HOST array of dim complex values; u,
  * state, workspace: device.  spin: NULL, or a device (n, 3) array that also
  * receives the Bloch vectors (as ss_spin) in the last pass. */
 int ss_state_scan(const double *psi0, const double *u, double *state, double *spin, int64_t n,
