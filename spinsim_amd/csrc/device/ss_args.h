@@ -65,6 +65,12 @@ struct ss_control {
 
 #define SS_MAX_CONTROLS 4
 
+// Adjoint workgroup size, and the gradient taps each lane keeps per control
+// in LDS: the launch gives the kernel n_controls * SS_ADJ_WINDOW *
+// SS_ADJ_BLOCK doubles of dynamic shared memory.
+#define SS_ADJ_WINDOW 8
+#define SS_ADJ_BLOCK 256
+
 // Batched adjoint launch: one lane per coarse row.  state[row] is the state at
 // the start of the row's interval, costate[row] the costate at its end.
 struct ss_adjoint_args {

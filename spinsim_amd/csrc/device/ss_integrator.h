@@ -1060,7 +1060,6 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
 #define SS_ADJ_TAYLOR 9      // exp(-A) v Taylor order
 #define SS_ADJ_NU 0.05       // sub-step until the generator norm bound is below this
 #define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 51)
-#define SS_ADJ_WINDOW 8      // gradient taps kept per lane and control in LDS
 
 struct ss_gen64 {
     double d[SS_D];
@@ -1280,7 +1279,7 @@ static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& 
             if (w[k] == 0.0 || idx < 0 || idx >= size) continue;
             const long long slot = idx - base[c];
             if (slot >= 0 && slot < SS_ADJ_WINDOW)
-                win[c * SS_ADJ_WINDOW + slot] += v * w[k];
+                win[(c * SS_ADJ_WINDOW + slot) * SS_ADJ_BLOCK] += v * w[k];
             else
                 atomicAdd(grad_row + ct.grad_offset + idx, v * w[k]);
         }
@@ -1309,13 +1308,16 @@ static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], ss_
 #endif
 }
 
-extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adjoint_args a)
+extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kernel(ss_adjoint_args a)
 {
-    __shared__ double windows[256][SS_MAX_CONTROLS * SS_ADJ_WINDOW];
+    // n_controls * SS_ADJ_WINDOW slots per lane, sized at launch; slot k of a
+    // lane's window is ss_adj_windows[k * SS_ADJ_BLOCK + lane], so a wave's
+    // accesses to one slot are consecutive doubles, free of LDS bank conflicts
+    extern __shared__ double ss_adj_windows[];
     const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double* win = windows[threadIdx.x];
+    double* win = ss_adj_windows + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < SS_MAX_CONTROLS * SS_ADJ_WINDOW; ++k) win[k] = 0.0;
+    for (int k = 0; k < a.n_controls * SS_ADJ_WINDOW; ++k) win[k * SS_ADJ_BLOCK] = 0.0;
     if (row >= a.total) return;
     long long lo = 0, hi = a.n_sims;
     while (hi - lo > 1) {
@@ -1470,7 +1472,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_adjoint_kernel(ss_adj
 #pragma unroll
         for (int s = 0; s < SS_ADJ_WINDOW; ++s) {
             const long long idx = base[c] + s;
-            const double v = win[c * SS_ADJ_WINDOW + s];
+            const double v = win[(c * SS_ADJ_WINDOW + s) * SS_ADJ_BLOCK];
             if (v != 0.0 && idx >= 0 && idx < size) atomicAdd(grad_row + a.controls[c].grad_offset + idx, v);
         }
     }
