@@ -219,8 +219,11 @@ def test_phased_launch(t1, dt, dt_out):
     assert_close(run_product(*args), ref)
 
 
-@pytest.mark.parametrize("squares", [0, 1, 7, 24])
+@pytest.mark.parametrize("squares", [0, 1, 7, 22, 23, 24, 30])
 def test_number_of_squares(squares):
+    """Spin-one Lie-Trotter at several square counts, either side of 12
+    hypercubes (22 squares: 11, squared in full; 23, 24 and 30: 12 and 15,
+    with the derived last row)."""
     print(f"case: number_of_squares={squares}", flush=True)
     psi0 = O.EIGENSTATES["one"]["plus_x"]
     args = (F.TILTED_ONE, 0, 1e-4, 1e-7, 1e-6, psi0)
