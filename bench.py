@@ -291,7 +291,7 @@ def batched_workload(ss, torch, steps, warmup, cpu_cores=None, n_sims=4096, chec
     sim = ss.Simulator(RABI_SWEEP, ss.SpinQuantumNumber.HALF, precision="fp32_bulk")
     psi0 = ss.SpinQuantumNumber.HALF.plus_z
     plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep, with_spin=True)
-    tune_batch(plan, reps=2, warmup=1, lanes=(1, 2, 4), items=(1, 4, 16))
+    tune_batch(plan, reps=2, warmup=1, lanes=(1, 2, 4), items=(1, 4, 8, 16))
     plan.capture()
     for _ in range(warmup):
         plan.replay()

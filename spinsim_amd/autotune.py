@@ -161,7 +161,7 @@ def tune_scan(plan, reps: int = 2, use_cache: bool = True, targets=(512, 1024, 2
     return best, table
 
 
-def tune_batch(plan, reps: int = 2, use_cache: bool = True, lanes=None, items=(1, 4, 16),
+def tune_batch(plan, reps: int = 2, use_cache: bool = True, lanes=None, items=(1, 4, 8, 16),
                warmup: int = 1, scan: bool = True):
     """Best (lanes_per_step, items_per_grab) for a BatchPlan, and with `scan`
     the state scan's chunk size (:func:`tune_scan`); applied to the plan.
