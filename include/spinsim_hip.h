@@ -73,9 +73,13 @@ enum { /* IntegrationMethod, :68-93 */
 };
 enum { /* arithmetic of the time-evolution kernel (no reference counterpart) */
     SS_PRECISION_FP64 = 0,     /* everything float64: the reference's arithmetic */
-    SS_PRECISION_FP32_BULK = 1 /* higher-order residuals of the step operators in
-                                  float32; samples, first-order terms, products
-                                  across lanes, outputs and scans float64 */
+    SS_PRECISION_FP32_BULK = 1, /* higher-order residuals of the step operators in
+                                   float32; samples, first-order terms, products
+                                   across lanes, outputs and scans float64 */
+    SS_PRECISION_FP64_SERIES = 2 /* the float32-bulk operations (residual series in
+                                    place of the exponentiation method) carried
+                                    out in float64: separates that algorithm's
+                                    speed from float32's */
 };
 enum { /* ExponentiationMethod, :95-129 */
     SS_EXPO_ANALYTIC = 0,
@@ -194,14 +198,18 @@ int ss_batch_final_state(const double *psi0, const double *u, const double *stat
 /* Adjoint gradient of a real loss of psi(T) with respect to the control
  * amplitudes, for a module whose field reads the controls' tables.  Needs the
  * forward batch's u and state and the costates; recomputes the fine steps.
- * grad (n_sims, grad_stride) and *status are cleared by the call; status
- * becomes 1 if a step's generator was too large for the derivative series.
- * (No reference counterpart: the reference has no gradients.) */
+ * tables: the device ss_table array the module's field reads (n_tables
+ * entries); each control reads table controls[c].table < n_tables and owns
+ * gradient entries [grad_offset, grad_offset + that table's size) of each row;
+ * entries at or beyond grad_stride are dropped.  grad (n_sims, grad_stride)
+ * and *status are cleared by the call; status becomes 1 if a step's generator
+ * was too large for the derivative series.  (No reference counterpart: the
+ * reference has no gradients.) */
 int ss_batch_adjoint(const ss_module *m, const double *sweep, int64_t sweep_stride, int64_t n_sims,
                      const int64_t *offsets, const double *t0, const double *dt,
                      const double *dt_out, const int32_t *n_fine, int64_t total, const double *u,
                      const double *state, const double *costate, const ss_table *tables,
-                     const ss_control_desc *controls, int n_controls, double *grad,
+                     int n_tables, const ss_control_desc *controls, int n_controls, double *grad,
                      int64_t grad_stride, int32_t *status, ss_stream stream);
 
 /* Evaluate interpolation table k (row `sim` when batched) at n device

@@ -182,8 +182,9 @@ class ControlledSimulator:
             mod.handle, plan.sweep.data_ptr(), plan.sweep.shape[1], b, plan.offsets.data_ptr(),
             plan.t0.data_ptr(), plan.dt.data_ptr(), plan.dt_out.data_ptr(),
             plan.n_fine.data_ptr(), plan.total, plan.u.data_ptr(), plan.state.data_ptr(),
-            costate.data_ptr(), self.sim._table_ptr(dev), ctypes.addressof(descs),
-            len(self.controls), grad.data_ptr(), self.grad_stride, status.data_ptr(), stream),
+            costate.data_ptr(), self.sim._table_ptr(dev), len(self.sim.field.samplers),
+            ctypes.addressof(descs), len(self.controls), grad.data_ptr(), self.grad_stride,
+            status.data_ptr(), stream),
             "ss_batch_adjoint")
         if int(status.item()) != 0:
             raise RuntimeError("ss_batch_adjoint: a step's rotation angle exceeds the derivative "

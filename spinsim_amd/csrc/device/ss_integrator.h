@@ -7,7 +7,10 @@
 //   SS_ROT 0|1          rotating frame
 //   SS_HYPER h          Lie-Trotter hypercubes (2h residual squarings)
 //   SS_CTX 0|1          callback takes the sampler context argument
-//   SS_PREC 0|1         plain float64 / float32 bulk (see ss_integrate_mixed)
+//   SS_PREC 0|1|2       plain float64 / float32 bulk / the float32-bulk series in
+//                       float64 (a precision ablation of the same arithmetic)
+//   SS_HOST             defined by the host (CPU device) build: the integrator
+//                       functions only, without the GPU kernels
 //
 // Each coarse (output) step i integrates n_fine sub-steps into U[i]
 // (spinsim/__init__.py:796-852).  A group of G lanes shares one coarse step:
@@ -447,17 +450,25 @@ static __device__ __forceinline__ void ss_append(const double g[SS_FIELD_LEN], s
     ss_left_multiply<SS_D>(e, u);
 }
 
-#if SS_PREC == 1
+#if SS_PREC != 0
 // ---- float32-bulk step operators -------------------------------------------
 // A step operator is E = exp(A), A = -i G(g).  Its first-order part A is kept
 // in float64 (as the generator vector g, summed exactly over a chunk of
 // steps); only the higher-order residual B = exp(A) - I - A and the products
 // of residuals are float32.  A chunk's product is I + A(sum g) + b with b
 // float32 of size |A|^2, so float32 rounding acts on second-order terms only.
+// SS_PREC 2 runs the same operations with ss_bulk = double.
+#if SS_PREC == 1
+typedef float ss_bulk;
+#define SS_BULK_FMA fmaf
+#else
+typedef double ss_bulk;
+#define SS_BULK_FMA fma
+#endif
 struct ss_cf {
-    float x, y;
+    ss_bulk x, y;
 };
-static __device__ __forceinline__ ss_cf ss_f(float x, float y)
+static __device__ __forceinline__ ss_cf ss_f(ss_bulk x, ss_bulk y)
 {
     ss_cf r;
     r.x = x;
@@ -466,19 +477,20 @@ static __device__ __forceinline__ ss_cf ss_f(float x, float y)
 }
 static __device__ __forceinline__ ss_cf ss_fmac(ss_cf acc, ss_cf a, ss_cf b)
 {
-    acc.x = fmaf(a.x, b.x, acc.x);
-    acc.x = fmaf(-a.y, b.y, acc.x);
-    acc.y = fmaf(a.x, b.y, acc.y);
-    acc.y = fmaf(a.y, b.x, acc.y);
+    acc.x = SS_BULK_FMA(a.x, b.x, acc.x);
+    acc.x = SS_BULK_FMA(-a.y, b.y, acc.x);
+    acc.y = SS_BULK_FMA(a.x, b.y, acc.y);
+    acc.y = SS_BULK_FMA(a.y, b.x, acc.y);
     return acc;
 }
 // i*d*z for real d
-static __device__ __forceinline__ ss_cf ss_fimul(float d, ss_cf z) { return ss_f(-d * z.y, d * z.x); }
+static __device__ __forceinline__ ss_cf ss_fimul(ss_bulk d, ss_cf z) { return ss_f(-d * z.y, d * z.x); }
 
-// A = -i G(g) as float: imaginary diagonal (0, dk), upper u = A[k][k+1],
-// lower l = A[k+1][k] (both constant along the off-diagonals), A[0][2] = 0.
+// A = -i G(g) in the bulk type: imaginary diagonal (0, dk), upper u =
+// A[k][k+1], lower l = A[k+1][k] (both constant along the off-diagonals),
+// A[0][2] = 0.
 struct ss_gen {
-    float d[SS_D];
+    ss_bulk d[SS_D];
     ss_cf u, l;
 };
 static __device__ __forceinline__ ss_gen ss_generator(const double* g, double scale)
@@ -487,19 +499,19 @@ static __device__ __forceinline__ ss_gen ss_generator(const double* g, double sc
 #if SS_DIM == 2
     // G = g.sigma/2
     const double hx = 0.5 * scale * g[0], hy = 0.5 * scale * g[1], hz = 0.5 * scale * g[2];
-    a.d[0] = (float)(-hz);
-    a.d[1] = (float)hz;
-    a.u = ss_f((float)(-hy), (float)(-hx));
-    a.l = ss_f((float)hy, (float)(-hx));
+    a.d[0] = (ss_bulk)(-hz);
+    a.d[1] = (ss_bulk)hz;
+    a.u = ss_f((ss_bulk)(-hy), (ss_bulk)(-hx));
+    a.l = ss_f((ss_bulk)hy, (ss_bulk)(-hx));
 #else
     // G = gx Jx + gy Jy + gz Jz + gq Q, Q = diag(1, -2, 1)/3
     const double rx = scale * g[0] * SS_INV_SQRT2, ry = scale * g[1] * SS_INV_SQRT2;
     const double z = scale * g[2], q3 = scale * g[3] * (1.0 / 3.0);
-    a.d[0] = (float)(-(z + q3));
-    a.d[1] = (float)(2.0 * q3);
-    a.d[2] = (float)(z - q3);
-    a.u = ss_f((float)(-ry), (float)(-rx));
-    a.l = ss_f((float)ry, (float)(-rx));
+    a.d[0] = (ss_bulk)(-(z + q3));
+    a.d[1] = (ss_bulk)(2.0 * q3);
+    a.d[2] = (ss_bulk)(z - q3);
+    a.u = ss_f((ss_bulk)(-ry), (ss_bulk)(-rx));
+    a.l = ss_f((ss_bulk)ry, (ss_bulk)(-rx));
 #endif
     return a;
 }
@@ -524,7 +536,7 @@ static __device__ __forceinline__ void ss_gen_apply(const ss_gen& a, const ss_cf
         for (int r = 0; r + 1 < SS_D; ++r) out[r][c] = ss_fmac(out[r][c], a.u, x[r + 1][c]);
 }
 
-// Dense out = x y (float32)
+// Dense out = x y (bulk type)
 static __device__ __forceinline__ void ss_fmatmul(const ss_cf x[SS_D][SS_D], const ss_cf y[SS_D][SS_D],
                                                   ss_cf out[SS_D][SS_D])
 {
@@ -543,14 +555,28 @@ static __device__ __forceinline__ void ss_fmatmul(const ss_cf x[SS_D][SS_D], con
             for (int c = 0; c < SS_D; ++c) out[r][c] = ss_fmac(out[r][c], x[r][k], y[k][c]);
 }
 
-// Dense float copy of A.
+// Scaling-and-squaring count of the spin-one float32 residual: the least s with
+// nu / 2^s <= 0.17.  Bounded for any input: a field sample that is infinite or
+// NaN gives 0 (the series then carries the inf/NaN into the result, as the
+// float64 exponentials do), and finite nu <= DBL_MAX needs at most 1,027.
+// (Tested on the host build of this function.)
+static __host__ __device__ __forceinline__ int ss_residual_squares(double nu)
+{
+    if (!(nu > 0.17) || !(nu <= 1.7976931348623157e308)) return 0;
+    // nu / 0.17 overflows beyond 0.17 DBL_MAX: there, log2(nu) - log2(0.17)
+    const double l = nu < 1e300 ? log2(nu / 0.17) : log2(nu) + 2.5563933485243853;
+    const int s = (int)ceil(l);
+    return s < 0 ? 0 : (s > 1100 ? 1100 : s);
+}
+
+// Dense bulk-type copy of A.
 static __device__ __forceinline__ void ss_gen_dense(const ss_gen& a, ss_cf m[SS_D][SS_D])
 {
 #pragma unroll
     for (int r = 0; r < SS_D; ++r)
 #pragma unroll
         for (int c = 0; c < SS_D; ++c)
-            m[r][c] = (r == c) ? ss_f(0.f, a.d[r]) : (c == r + 1) ? a.u : (r == c + 1) ? a.l : ss_f(0.f, 0.f);
+            m[r][c] = (r == c) ? ss_f((ss_bulk)0, a.d[r]) : (c == r + 1) ? a.u : (r == c + 1) ? a.l : ss_f((ss_bulk)0, (ss_bulk)0);
 }
 
 #if SS_DIM == 2
@@ -562,20 +588,26 @@ static __device__ __forceinline__ void ss_gen_dense(const ss_gen& a, ss_cf m[SS_
 static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b[SS_D][SS_D])
 {
     const double th2 = 0.25 * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-    float c1, c2;
+    ss_bulk c1, c2;
     if (th2 <= 0.25) {
-        const float t2 = (float)th2;
-        c1 = t2 * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
-                                4.1666668e-2f), -0.5f);
-        c2 = t2 * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.5052108e-8f, 2.7557319e-6f), -1.9841270e-4f),
-                                8.3333333e-3f), -0.16666667f);
+        const ss_bulk t2 = (ss_bulk)th2;
+        c1 = t2 * SS_BULK_FMA(t2, SS_BULK_FMA(t2, SS_BULK_FMA(t2, SS_BULK_FMA(t2, (ss_bulk)-2.7557319223985893e-07,
+                                                                     (ss_bulk)2.48015873015873e-05),
+                                                       (ss_bulk)-1.388888888888889e-03),
+                                         (ss_bulk)4.1666666666666664e-02),
+                              (ss_bulk)-0.5);
+        c2 = t2 * SS_BULK_FMA(t2, SS_BULK_FMA(t2, SS_BULK_FMA(t2, SS_BULK_FMA(t2, (ss_bulk)-2.505210838544172e-08,
+                                                                     (ss_bulk)2.7557319223985893e-06),
+                                                       (ss_bulk)-1.984126984126984e-04),
+                                         (ss_bulk)8.333333333333333e-03),
+                              (ss_bulk)-0.16666666666666666);
     } else {
         const double th = sqrt(th2);
         double s, c, sh, ch;
         sincos(th, &s, &c);
         sincos(0.5 * th, &sh, &ch);
-        c1 = (float)(-2.0 * sh * sh);
-        c2 = (float)(s / th - 1.0);
+        c1 = (ss_bulk)(-2.0 * sh * sh);
+        c2 = (ss_bulk)(s / th - 1.0);
     }
     const ss_gen a = ss_generator(g, 1.0);
     b[0][0] = ss_f(c1, c2 * a.d[0]);
@@ -591,18 +623,14 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
 static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b[SS_D][SS_D])
 {
     const double nu = fabs(g[0]) + fabs(g[1]) + fabs(g[2]) + fabs(g[3]);
-    int squares = 0;
-    double scale = 1.0;
-    if (nu > 0.17) {
-        squares = (int)ceil(log2(nu / 0.17));
-        scale = ldexp(1.0, -squares);
-    }
+    const int squares = ss_residual_squares(nu);
+    const double scale = ldexp(1.0, -squares);
     const ss_gen a = ss_generator(g, scale);
     ss_cf t[SS_D][SS_D], w[SS_D][SS_D];
     const bool short_series = nu <= 0.015;
     // t = c_top A + c_next I
-    const float c_top = short_series ? (1.0f / 24.0f) : (1.0f / 720.0f);
-    const float c_next = short_series ? (1.0f / 6.0f) : (1.0f / 120.0f);
+    const ss_bulk c_top = short_series ? (ss_bulk)(1.0 / 24.0) : (ss_bulk)(1.0 / 720.0);
+    const ss_bulk c_next = short_series ? (ss_bulk)(1.0 / 6.0) : (ss_bulk)(1.0 / 120.0);
     ss_gen_dense(a, t);
 #pragma unroll
     for (int r = 0; r < SS_D; ++r)
@@ -613,7 +641,7 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
             if (r == c) t[r][c].x += c_next;
         }
     if (!short_series) {
-        const float cs[3] = {1.0f / 24.0f, 1.0f / 6.0f, 0.5f};
+        const ss_bulk cs[3] = {(ss_bulk)(1.0 / 24.0), (ss_bulk)(1.0 / 6.0), (ss_bulk)0.5};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             ss_gen_apply(a, t, w);
@@ -627,7 +655,7 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
 #pragma unroll
         for (int r = 0; r < SS_D; ++r)
 #pragma unroll
-            for (int c = 0; c < SS_D; ++c) t[r][c] = (r == c) ? ss_f(w[r][c].x + 0.5f, w[r][c].y) : w[r][c];
+            for (int c = 0; c < SS_D; ++c) t[r][c] = (r == c) ? ss_f(w[r][c].x + (ss_bulk)0.5, w[r][c].y) : w[r][c];
     }
     ss_gen_apply(a, t, w);   // A T
     ss_gen_apply(a, w, b);   // A A T
@@ -636,7 +664,7 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
         // X = A_k + B_k, A_k = 2^k A_s; new B = 2 B + X^2
         ss_cf x[SS_D][SS_D];
         ss_gen_dense(a, x);
-        const float f = ldexpf(1.0f, k);
+        const ss_bulk f = (ss_bulk)ldexp(1.0, k);
 #pragma unroll
         for (int r = 0; r < SS_D; ++r)
 #pragma unroll
@@ -645,7 +673,7 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
 #pragma unroll
         for (int r = 0; r < SS_D; ++r)
 #pragma unroll
-            for (int c = 0; c < SS_D; ++c) b[r][c] = ss_f(2.f * b[r][c].x + w[r][c].x, 2.f * b[r][c].y + w[r][c].y);
+            for (int c = 0; c < SS_D; ++c) b[r][c] = ss_f((ss_bulk)2 * b[r][c].x + w[r][c].x, (ss_bulk)2 * b[r][c].y + w[r][c].y);
     }
 }
 
@@ -735,7 +763,7 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #pragma unroll 1
     for (int j = 0; j < j0; ++j) tf += dt;
     ss_set_identity<SS_D>(u);
-#if SS_PREC == 1
+#if SS_PREC != 0
     double gsum[SS_LIE];
     ss_cf bres[SS_D][SS_D];
 #pragma unroll
@@ -743,7 +771,7 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #pragma unroll
     for (int r = 0; r < SS_D; ++r)
 #pragma unroll
-        for (int c = 0; c < SS_D; ++c) bres[r][c] = ss_f(0.f, 0.f);
+        for (int c = 0; c < SS_D; ++c) bres[r][c] = ss_f((ss_bulk)0, (ss_bulk)0);
 #define SS_APPEND(g) ss_append_mixed(g, gsum, bres)
 #else
 #define SS_APPEND(g) ss_append(g, u)
@@ -801,7 +829,7 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #endif
         }
     }
-#if SS_PREC == 1
+#if SS_PREC != 0
     ss_mixed_to_u(gsum, bres, u);
 #endif
 #undef SS_APPEND
@@ -831,6 +859,7 @@ static __device__ __forceinline__ double ss_rotating_wave(double tc, double dt_o
     return rot;
 }
 
+#ifndef SS_HOST
 // Ordered combine of the G lanes of a group: rank r (a multiple of 2s) takes
 // the product of the later ranks from lane r + s, u_r <- u_{r+s} u_r.
 static __device__ __forceinline__ void ss_group_reduce(ss_cd u[SS_D][SS_D], int rank, int lanes)
@@ -848,8 +877,9 @@ static __device__ __forceinline__ void ss_group_reduce(ss_cd u[SS_D][SS_D], int 
         if ((rank & (2 * s - 1)) == 0 && rank + s < lanes) ss_left_multiply<SS_D>(p, u);
     }
 }
+#endif  // SS_HOST
 
-#if SS_PREC == 1
+#if SS_PREC != 0
 // Re-unitarisation of a float32-bulk coarse operator: Newton-Schulz steps
 // towards the unitary polar factor, U <- U (3I - U^+ U) / 2 = U - U E / 2 with
 // E = U^+ U - I, in float64.  A step maps a deviation max|E| = e to about
@@ -892,7 +922,7 @@ static __device__ __forceinline__ void ss_reunitarise(ss_cd u[SS_D][SS_D])
 static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot, double dt_out, double tc,
                                                 long long row, double* time_out, double* u_out)
 {
-#if SS_PREC == 1
+#if SS_PREC != 0
     ss_reunitarise(u);
 #endif
 #if SS_ROT
@@ -914,6 +944,8 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
     time_out[row] = tc;
 }
 
+// ---- GPU kernels (the host build of this text, for the CPU devices, stops here)
+#ifndef SS_HOST
 // One item of a single-simulation launch (get_time_evolution,
 // spinsim/__init__.py:854-896): in phase p, whose items start at item number
 // `first`, group gi of `lanes` lanes integrates coarse row
@@ -1280,7 +1312,7 @@ static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& 
             const long long slot = idx - base[c];
             if (slot >= 0 && slot < SS_ADJ_WINDOW)
                 win[(c * SS_ADJ_WINDOW + slot) * SS_ADJ_BLOCK] += v * w[k];
-            else
+            else if (ct.grad_offset + idx < a.grad_stride)
                 atomicAdd(grad_row + ct.grad_offset + idx, v * w[k]);
         }
     }
@@ -1473,7 +1505,9 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
         for (int s = 0; s < SS_ADJ_WINDOW; ++s) {
             const long long idx = base[c] + s;
             const double v = win[(c * SS_ADJ_WINDOW + s) * SS_ADJ_BLOCK];
-            if (v != 0.0 && idx >= 0 && idx < size) atomicAdd(grad_row + a.controls[c].grad_offset + idx, v);
+            if (v != 0.0 && idx >= 0 && idx < size && a.controls[c].grad_offset + idx < a.grad_stride)
+                atomicAdd(grad_row + a.controls[c].grad_offset + idx, v);
         }
     }
 }
+#endif  // SS_HOST

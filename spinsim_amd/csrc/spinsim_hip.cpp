@@ -20,6 +20,8 @@
 #include "spinsim_hip.h"
 #include "device/ss_args.h"
 #include "device/ss_complex.h"
+#include "device/ss_bloch.h"
+#include "host/ss_offsets.h"
 #include "ss_embedded.h"
 
 namespace dev {  // the sampler as field callbacks see it (same layout as ::ss_table)
@@ -47,9 +49,13 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 6;
+constexpr int kAbiVersion = 7;
 static_assert(sizeof(::ss_table) == 40, "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
+
+// hipRTC options of every integrator compile (packed FP32 instructions off).
+const char* kRtcOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                            "-Xclang", "-packed-fp32-ops", "-Wno-pragma-once-outside-header"};
 
 uint64_t fnv1a(const char* s, uint64_t h)
 {
@@ -124,30 +130,6 @@ __device__ __forceinline__ void store_vec(double* s, long long k, const ss_cd v[
     for (int r = 0; r < D; ++r) p[r] = make_double2(v[r].x, v[r].y);
 }
 
-// get_spin, spinsim/__init__.py:936-944, for one state.
-template <int D>
-__device__ __forceinline__ void store_spin(const ss_cd s[D], double* spin, long long i)
-{
-    double sx, sy, sz;
-    if (D == 2) {
-        const ss_cd p = ss_mul(s[0], ss_conj(s[1]));
-        sx = p.x;
-        sy = -p.y;  // Re(i psi0 psi1*)
-        sz = 0.5 * (s[0].x * s[0].x + s[0].y * s[0].y - s[1].x * s[1].x - s[1].y * s[1].y);
-    } else {
-        const double r2 = 1.4142135623730951;
-        const ss_cd c1 = ss_conj(s[1]);
-        const ss_cd a = ss_mul(c1, ss_add(s[0], s[D - 1]));
-        const ss_cd b = ss_mul(c1, ss_c(s[0].x - s[D - 1].x, s[0].y - s[D - 1].y));
-        sx = 2.0 * a.x / r2;
-        sy = -2.0 * b.y / r2;  // Re(2i b)/sqrt2
-        sz = s[0].x * s[0].x + s[0].y * s[0].y - s[D - 1].x * s[D - 1].x - s[D - 1].y * s[D - 1].y;
-    }
-    spin[3 * i + 0] = sx;
-    spin[3 * i + 1] = sy;
-    spin[3 * i + 2] = sz;
-}
-
 template <int D>
 __global__ void __launch_bounds__(256) spin_kernel(const double* state, double* spin, long long n)
 {
@@ -155,7 +137,7 @@ __global__ void __launch_bounds__(256) spin_kernel(const double* state, double* 
     if (i >= n) return;
     ss_cd s[D];
     load_vec<D>(state, i, s);
-    store_spin<D>(s, spin, i);
+    ss_store_spin<D>(s, spin, i);
 }
 
 // ---- state scan of one simulation: three passes ---------------------------
@@ -298,7 +280,7 @@ __global__ void __launch_bounds__(kSingleScanBlock) single_scan_apply(const doub
     ss_matvec<D>(pre, c, v);
     for (long long k = k0; k < kend; ++k) {
         store_vec<D>(state, k, v);
-        if (spin) store_spin<D>(v, spin, k);
+        if (spin) ss_store_spin<D>(v, spin, k);
         if (k + 1 < kend) {
             ss_cd m[D][D], t[D];
             load_mat<D>(u, k, m);
@@ -357,7 +339,7 @@ __global__ void __launch_bounds__(kSingleScanBlock) single_scan_apply_carry(cons
     ss_matvec<D>(pre, c, v);
     for (long long k = k0; k < kend; ++k) {
         store_vec<D>(state, k, v);
-        if (spin) store_spin<D>(v, spin, k);
+        if (spin) ss_store_spin<D>(v, spin, k);
         if (k + 1 < kend) {
             ss_cd mm[D][D], tv[D];
             load_mat<D>(u, k, mm);
@@ -878,7 +860,7 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
         }
         const long long o = REV ? a.total - 1 - r : r;
         store_vec<D>(a.out, o, v);
-        if (a.spin) store_spin<D>(v, a.spin, o);
+        if (a.spin) ss_store_spin<D>(v, a.spin, o);
         if (r == r1 - 1) {
 #pragma unroll
             for (int i = 0; i < D; ++i) s_v[i] = v[i];
@@ -1073,7 +1055,14 @@ uint64_t ss_source_hash(void)
     h = fnv1a(ss_src_complex, h);
     h = fnv1a(ss_src_prelude, h);
     h = fnv1a(ss_src_integrator, h);
-    return h;
+    // the compiler options and the hipRTC version make a different code
+    // object from the same text
+    for (const char* o : kRtcOptions) h = fnv1a(o, fnv1a("\x1f", h));
+    int major = 0, minor = 0;
+    hiprtcVersion(&major, &minor);
+    char ver[32];
+    snprintf(ver, sizeof(ver), "hiprtc %d.%d", major, minor);
+    return fnv1a(ver, h);
 }
 
 void ss_free(void* p) { std::free(p); }
@@ -1092,11 +1081,12 @@ int ss_compile_code(const char* field_src, int dim, int method, int expo, int ro
     if (expo == 0 && dim != 2) return fail("ss_compile_code: analytic exponentiation is spin-half only");
     if (expo == SS_EXPO_LIE_TROTTER_8 && dim != 3)
         return fail("ss_compile_code: eight-generator Lie-Trotter exponentiation is spin-one only");
-    if (expo == SS_EXPO_LIE_TROTTER_8 && precision == SS_PRECISION_FP32_BULK)
+    if (expo == SS_EXPO_LIE_TROTTER_8 && precision != SS_PRECISION_FP64)
         return fail("ss_compile_code: the float32-bulk mode covers x, y, z and q only, not the "
                     "eight-generator exponentiation");
     if (number_of_squares > 60) return fail("ss_compile_code: number_of_squares %d too large", number_of_squares);
-    if (precision != SS_PRECISION_FP64 && precision != SS_PRECISION_FP32_BULK)
+    if (precision != SS_PRECISION_FP64 && precision != SS_PRECISION_FP32_BULK &&
+        precision != SS_PRECISION_FP64_SERIES)
         return fail("ss_compile_code: bad precision %d", precision);
     const std::string src = build_source(field_src, dim, method, expo, rotating, number_of_squares,
                                          field_has_context, precision);
@@ -1104,9 +1094,7 @@ int ss_compile_code(const char* field_src, int dim, int method, int expo, int ro
     if (hiprtcCreateProgram(&prog, src.c_str(), "spinsim_field.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
         return fail("hiprtcCreateProgram failed");
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
-                          "-Xclang", "-packed-fp32-ops", "-Wno-pragma-once-outside-header"};
-    hiprtcResult rc = hiprtcCompileProgram(prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+    hiprtcResult rc = hiprtcCompileProgram(prog, int(sizeof(kRtcOptions) / sizeof(kRtcOptions[0])), kRtcOptions);
     size_t log_size = 0;
     hiprtcGetProgramLogSize(prog, &log_size);
     std::string text(log_size, '\0');
@@ -1203,24 +1191,6 @@ int ss_module_info(const ss_module* m, int* vgprs, int* sgprs, int* scratch_byte
     return 0;
 }
 
-static void sample_offsets(int method, double dt, double* off_a, double* off_b)
-{
-    // Sample points within a fine step, in the reference's operation order:
-    // CF4 t + 0.5*dt*(1 -+ 1/sqrt3) (:644, :649), midpoint t + 0.5*dt (:771),
-    // half step t and t + dt (:713, :718).
-    const double sqrt3 = 1.7320508075688772;
-    if (method == SS_METHOD_MAGNUS_CF4) {
-        *off_a = 0.5 * dt * (1 - 1 / sqrt3);
-        *off_b = 0.5 * dt * (1 + 1 / sqrt3);
-    } else if (method == SS_METHOD_MIDPOINT_SAMPLE) {
-        *off_a = 0.5 * dt;
-        *off_b = 0.0;
-    } else {
-        *off_a = 0.0;
-        *off_b = dt;
-    }
-}
-
 // Phase table for a single simulation on `waves` waves.  Candidate lanes per
 // coarse step G are the divisors of n_fine up to 64 (and the powers of two
 // when n_fine > 64, where no G gives one step per lane); an
@@ -1312,7 +1282,7 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
     a.t0 = t0;
     a.dt = dt;
     a.dt_out = dt_out;
-    sample_offsets(m->method, dt, &a.off_a, &a.off_b);
+    ss_sample_offsets(m->method, dt, &a.off_a, &a.off_b);
     a.n_fine = n_fine;
     if (lanes_per_step > 0) {
         const long long groups = 64 / lanes_per_step;
@@ -1377,7 +1347,7 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
     a.n_sims = n_sims;
     a.sweep_stride = sweep_stride;
     double oa = 0, ob = 0;
-    sample_offsets(m->method, 2.0, &oa, &ob);  // offsets in units of 0.5*dt
+    ss_sample_offsets(m->method, 2.0, &oa, &ob);  // offsets in units of 0.5*dt
     a.off_a_frac = oa;
     a.off_b_frac = ob;
     a.lanes_per_step = lanes_per_step;
@@ -1558,8 +1528,9 @@ int ss_batch_final_state(const double* psi0, const double* u, const double* stat
 int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stride, int64_t n_sims,
                      const int64_t* offsets, const double* t0, const double* dt, const double* dt_out,
                      const int32_t* n_fine, int64_t total, const double* u, const double* state,
-                     const double* costate, const ss_table* tables, const ss_control_desc* controls,
-                     int n_controls, double* grad, int64_t grad_stride, int32_t* status, ss_stream stream)
+                     const double* costate, const ss_table* tables, int n_tables,
+                     const ss_control_desc* controls, int n_controls, double* grad, int64_t grad_stride,
+                     int32_t* status, ss_stream stream)
 {
     if (!m) return fail("ss_batch_adjoint: null module");
     if (m->expo == SS_EXPO_LIE_TROTTER_8)
@@ -1572,7 +1543,11 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
         const int ax = controls[c].axis;
         if (ax < 0 || ax > 3 || (m->dim == 2 && ax == 3))
             return fail("ss_batch_adjoint: control %d has axis %d (x=0, y=1, z=2, q=3 for spin one)", c, ax);
-        if (controls[c].table < 0) return fail("ss_batch_adjoint: control %d has a negative table", c);
+        if (controls[c].table < 0 || controls[c].table >= n_tables)
+            return fail("ss_batch_adjoint: control %d reads table %d of %d", c, controls[c].table, n_tables);
+        if (controls[c].grad_offset < 0 || controls[c].grad_offset >= grad_stride)
+            return fail("ss_batch_adjoint: control %d has gradient offset %lld outside a row of %lld", c,
+                        (long long)controls[c].grad_offset, (long long)grad_stride);
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     const long long n_grad = (long long)n_sims * grad_stride;
@@ -1601,7 +1576,7 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
     a.sweep_stride = sweep_stride;
     a.grad_stride = grad_stride;
     double oa = 0, ob = 0;
-    sample_offsets(m->method, 2.0, &oa, &ob);
+    ss_sample_offsets(m->method, 2.0, &oa, &ob);
     a.off_a_frac = oa;
     a.off_b_frac = ob;
     a.n_controls = n_controls;

@@ -102,10 +102,40 @@ class InterpolationSampler:
         rec.zero_boundary = int(self.zero_boundary)
         return rec
 
+    def host_record(self):
+        """(ss_table record, arrays it points at) for the CPU devices; the
+        caller keeps the arrays alive while the record is in use."""
+        if self._bound is not None:
+            raise RuntimeError("a sampler bound to device amplitudes has no host table")
+        arrays = (np.ascontiguousarray(self.time), np.ascontiguousarray(self.amplitude))
+        rec = SsTable()
+        rec.time = arrays[0].ctypes.data
+        rec.amplitude = arrays[1].ctypes.data
+        rec.size = self.time.size
+        rec.amplitude_stride = self.time.size if self.amplitude.ndim == 2 else 0
+        rec.order = self.order
+        rec.zero_boundary = int(self.zero_boundary)
+        return rec, arrays
+
+    def evaluate_host(self, time_sample, row: int = 0):
+        """Evaluate on the host (the host build of the device sampler)."""
+        from . import host
+
+        t = np.ascontiguousarray(np.atleast_1d(np.asarray(time_sample, np.float64)))
+        out = np.empty_like(t)
+        arr, _keep = build_host_table_array([self])
+        host._check(host.host_lib().ss_host_interp_eval(ctypes.addressof(arr), 0, int(row),
+                                                        t.ctypes.data, out.ctypes.data, t.size),
+                    "ss_host_interp_eval")
+        return out[0] if np.ndim(time_sample) == 0 else out.reshape(np.shape(time_sample))
+
     def __call__(self, time_sample, row: int = 0):
-        """Evaluate on the GPU at the given time(s), like the reference's host sampler."""
+        """Evaluate at the given time(s), like the reference's host-callable
+        sampler: on the HIP device when one is visible, else on the host."""
         import torch
 
+        if not torch.cuda.is_available():
+            return self.evaluate_host(time_sample, row)
         device = torch.device("cuda", torch.cuda.current_device())
         t = torch.as_tensor(np.atleast_1d(np.asarray(time_sample, np.float64)), device=device)
         out = torch.empty_like(t)
@@ -131,6 +161,17 @@ def build_table_array(samplers, device):
     dev = host.to(device)
     dev._ss_keep = keep  # the tables point into these tensors
     return dev
+
+
+def build_host_table_array(samplers):
+    """(ctypes ss_table array, the host arrays its records point at) for the
+    CPU devices; keep both while the array is in use."""
+    arr = (SsTable * max(len(samplers), 1))()
+    keep = []
+    for k, s in enumerate(samplers):
+        arr[k], arrays = s.host_record()
+        keep.append(arrays)
+    return arr, keep
 
 
 def generate_interpolation_sampler(time, amplitude, order=3, zero_boundary=True, device=None):

@@ -9,10 +9,12 @@ and returns numpy arrays like the reference, or device tensors on request.
 """
 from __future__ import annotations
 
+import ctypes
 import hashlib
 import math
 import os
 import threading
+import warnings
 
 import numpy as np
 
@@ -36,7 +38,7 @@ def _cache_dir():
     return d
 
 
-PRECISIONS = {"fp64": 0, "fp32_bulk": 1}
+PRECISIONS = {"fp64": 0, "fp32_bulk": 1, "fp64_series": 2}
 
 
 def compile_field(src: str, dim: int, method: int, expo: int, rotating: bool,
@@ -144,13 +146,18 @@ class Results:
 
 
 def _default_device():
+    """Device.ROCM when a HIP device is visible, else Device.CPU (as the
+    reference picks CUDA when available, else CPU, :435-439)."""
     try:
         import torch
         if torch.cuda.is_available():
             return Device.ROCM
     except ImportError:
         pass
-    return Device.ROCM  # construction still compiles; evaluation needs a HIP device
+    return Device.CPU
+
+
+HOST_DEVICES = (Device.CPU, Device.CPU_SINGLE, Device.PYTHON)
 
 
 def resolve_exponentiation(spin_quantum_number, exponentiation_method, warn=True):
@@ -203,8 +210,19 @@ class Simulator:
     lanes, the outputs and the state scan stay float64.  Accurate when each
     step's rotation angle is small (rotating frame); check with
     :func:`spinsim_amd.precision_error`.
-    ``threads_per_block``, ``max_registers`` and ``number_of_threads`` are
-    accepted for compatibility and do not change the gfx950 launch.
+    ``threads_per_block`` and ``max_registers`` are accepted for
+    compatibility and do not change the gfx950 launch.
+
+    Devices: ``Device.ROCM`` (alias ``ROC``) runs the gfx950 HIP kernels; the
+    default when a HIP device is visible.  ``Device.CPU`` runs a host build of
+    the same integrator text (g++, OpenMP over the coarse steps like the
+    reference's ``nb.prange``, :883-885) on ``number_of_threads`` threads (all
+    host cores when None, :1035-1047); the default without a HIP device.
+    ``Device.CPU_SINGLE`` runs it on one thread, and ``Device.PYTHON`` -- in
+    the reference the uncompiled Python integrator -- runs it on one thread
+    too.  ``Device.CUDA`` is not supported.  The CPU devices compute
+    ``evaluate``, ``get_time_evolution`` and ``spin_calculator``; batches and
+    gradients run on ROCm devices.
     """
 
     def __init__(self, get_field, spin_quantum_number: SpinQuantumNumber, device: Device = None,
@@ -215,10 +233,10 @@ class Simulator:
                  precision: str = "fp64"):
         if not device:
             device = _default_device()
-        if device != Device.ROCM:
+        if device == Device.CUDA:
             raise NotImplementedError(
-                f"{device} is a numba back end of the reference; spinsim_amd runs on ROCm "
-                "devices (Device.ROCM)")
+                "Device.CUDA is the reference's numba.cuda back end; spinsim_amd runs on ROCm "
+                "devices (Device.ROCM) and on the host (Device.CPU, CPU_SINGLE, PYTHON)")
         self.threads_per_block = threads_per_block
         self.spin_quantum_number = spin_quantum_number
         self.device = device
@@ -232,24 +250,93 @@ class Simulator:
         self.precision = precision
         self.exponentiation_method = resolve_exponentiation(spin_quantum_number,
                                                             exponentiation_method)
-        if precision == "fp32_bulk" and self.exponentiation_method == ExponentiationMethod.LIE_TROTTER_8:
-            raise ValueError("precision='fp32_bulk' covers the x, y, z and q generators only; "
+        if precision != "fp64" and self.exponentiation_method == ExponentiationMethod.LIE_TROTTER_8:
+            raise ValueError(f"precision={precision!r} covers the x, y, z and q generators only; "
                              "use precision='fp64' with ExponentiationMethod.LIE_TROTTER_8")
+        # what computes each step operator (the residual series replaces the
+        # exponentiation method in the float32-bulk modes)
+        if precision == "fp64":
+            self.step_exponential = self.exponentiation_method.value
+        else:
+            self.step_exponential = ("residual series, " +
+                                     ("float32" if precision == "fp32_bulk" else "float64"))
+            if exponentiation_method is not None or number_of_squares != 24:
+                warnings.warn(
+                    f"precision={precision!r} computes each step operator as exp(A) = I + A + "
+                    "a Taylor residual series (scaled and squared when large); "
+                    f"exponentiation_method={self.exponentiation_method.name} and "
+                    f"number_of_squares={number_of_squares} are not used", stacklevel=2)
         self.dimension = spin_quantum_number.dimension
         self._modules = {}
         self._tables = {}
+        self._host = device in HOST_DEVICES
         try:
             self.field = FieldSource(get_field, samplers)
-            self._code = compile_field(self.field.source, self.dimension,
-                                       integration_method.index, self.exponentiation_method.index,
-                                       self.use_rotating_frame, self.number_of_squares,
-                                       self.field.has_context, PRECISIONS[precision])
+            if self._host:
+                from . import host
+                self._code = None
+                self._host_module = host.compile_module(
+                    self.field.source, self.dimension, integration_method.index,
+                    self.exponentiation_method.index, self.use_rotating_frame,
+                    self.number_of_squares, self.field.has_context, PRECISIONS[precision])
+            else:
+                self._code = compile_field(self.field.source, self.dimension,
+                                           integration_method.index,
+                                           self.exponentiation_method.index,
+                                           self.use_rotating_frame, self.number_of_squares,
+                                           self.field.has_context, PRECISIONS[precision])
         except Exception:
-            print(f"{_RED}spinsim error!!!\nhipRTC could not compile get_field() into a device "
+            compiler = "g++" if self._host else "hipRTC"
+            print(f"{_RED}spinsim error!!!\n{compiler} could not compile get_field() into a device "
                   f"function.{_RESET}\n")
             raise
         self.get_time_evolution = self._get_time_evolution
         self.spin_calculator = self._spin_calculator
+
+    # ---- host (CPU device) plumbing ----------------------------------------
+    def _host_threads(self):
+        """Threads of a CPU-device call: number_of_threads (all host cores when
+        None) for Device.CPU, one for CPU_SINGLE and PYTHON."""
+        if self.device != Device.CPU:
+            return 1
+        if self.number_of_threads:
+            return int(self.number_of_threads)
+        return len(os.sched_getaffinity(0))
+
+    def _host_tables(self):
+        if not self.field.samplers:
+            return None
+        versions = tuple(s.version for s in self.field.samplers)
+        cached = self._tables.get("host")
+        if cached is None or cached[0] != versions:
+            from .interpolation import build_host_table_array
+            cached = (versions, build_host_table_array(self.field.samplers))
+            self._tables["host"] = cached
+        return ctypes.addressof(cached[1][0])  # (array, arrays it points at)
+
+    def _host_time_evolution(self, sweep_parameters, time_coarse, time_end_points,
+                             time_step_integration, time_step_output, time_evolution_output):
+        t0 = float(np.asarray(time_end_points, np.float64)[0])
+        self._host_module.time_evolution(sweep_parameters, self._host_tables(), t0,
+                                         time_step_integration, time_step_output, time_coarse,
+                                         time_evolution_output, self._host_threads())
+
+    def _evaluate_host(self, time_start, time_end, time_step_integration, time_step_output,
+                       state_init, sweep_parameters, output):
+        from . import host
+        dt, n, ends = derive_steps(time_start, time_end, time_step_integration, time_step_output)
+        if n < 0:
+            raise ValueError("time_end is before time_start")
+        d = self.dimension
+        time = np.empty(n, np.float64)
+        u = np.empty((n, d, d), np.complex128)
+        self._host_time_evolution(sweep_parameters, time, ends, dt, time_step_output, u)
+        state, _ = host.state(state_init, u)
+        if output == "torch":
+            import torch
+            return Results(torch.from_numpy(time), torch.from_numpy(u), torch.from_numpy(state),
+                           self.spin_calculator)
+        return Results(time, u, state, self.spin_calculator)
 
     # ---- device plumbing --------------------------------------------------
     @staticmethod
@@ -260,6 +347,9 @@ class Simulator:
         return torch
 
     def _module(self, device):
+        if self._host:
+            raise RuntimeError(f"this Simulator runs on {self.device}; device launches need "
+                               "Device.ROCM")
         key = device.index
         if key not in self._modules:
             torch = self._torch()
@@ -338,8 +428,13 @@ class Simulator:
 
         ``time_coarse`` (N,) float64 and ``time_evolution_output`` (N, d, d)
         complex128 are device tensors filled in place (numpy arrays are copied
-        through the device).
+        through the device); on the CPU devices they are numpy arrays.
         """
+        if self._host:
+            self._host_time_evolution(sweep_parameters, time_coarse, time_end_points,
+                                      time_step_integration, time_step_output,
+                                      time_evolution_output)
+            return None
         torch = self._torch()
         out_np = isinstance(time_evolution_output, np.ndarray)
         if out_np:
@@ -377,6 +472,11 @@ class Simulator:
 
     def _spin_calculator(self, state):
         """Bloch vectors of a state time series (spin_calculator, :961-988)."""
+        if self._host:
+            from . import host
+            if not isinstance(state, np.ndarray):
+                return torch_from_host(host.spin(state.cpu().numpy(), self._host_threads()))
+            return host.spin(state, self._host_threads())
         torch = self._torch()
         host = isinstance(state, np.ndarray)
         if host:
@@ -395,6 +495,9 @@ class Simulator:
         """Integrate and return the state time series (spinsim/__init__.py:993-1085)."""
         if output not in ("numpy", "torch"):
             raise ValueError("output must be 'numpy' or 'torch'")
+        if self._host:
+            return self._evaluate_host(time_start, time_end, time_step_integration,
+                                       time_step_output, state_init, sweep_parameters, output)
         torch = self._torch()
         dt, n, ends = derive_steps(time_start, time_end, time_step_integration, time_step_output)
         if n < 0:
@@ -444,9 +547,16 @@ class Simulator:
         Any of the times/steps may be per-simulation (B,) arrays and
         ``state_init`` may be (B, d); returns :class:`spinsim_amd.batch.BatchResults`.
         """
+        if self._host:
+            raise NotImplementedError("batched evaluation runs on ROCm devices (Device.ROCM)")
         from .batch import evaluate_batch
         return evaluate_batch(self, time_start, time_end, time_step_integration,
                               time_step_output, state_init, sweep_parameters, **kw)
+
+
+def torch_from_host(a):
+    import torch
+    return torch.from_numpy(a)
 
 
 def precision_error(sim: Simulator, time_start, time_end, time_step_integration, time_step_output,

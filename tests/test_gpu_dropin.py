@@ -148,3 +148,13 @@ def test_numpy_results_staged_through_pinned_memory():
         assert np.array_equal(getattr(a, name), getattr(c, name)), name
     empty = sim.evaluate(0.0, 0.0, 1e-7, 1e-6, O.EIGENSTATES["one"]["plus_x"])
     assert empty.state.shape == (0, 3)
+
+
+def test_default_device_is_rocm_with_a_hip_device():
+    """With a HIP device visible the default device is ROCm (the GPU tests
+    that construct default simulators run the HIP kernels, never the host
+    build)."""
+    import spinsim_amd as ss
+
+    sim = ss.Simulator(F.LARMOR.src, ss.SpinQuantumNumber.HALF)
+    assert sim.device == ss.Device.ROCM and not sim._host

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == _native.ABI_VERSION == 6
+    assert lib.ss_abi_version() == _native.ABI_VERSION == 7
     assert lib.ss_source_hash() != 0
 
 
@@ -121,9 +121,9 @@ def test_enums_match_reference():
     assert {d.name for d in ss.Device} >= {"PYTHON", "CPU_SINGLE", "CPU", "CUDA", "ROCM"}
 
 
-def test_non_rocm_devices_raise():
+def test_cuda_device_raises():
     with pytest.raises(NotImplementedError):
-        ss.Simulator(F.LARMOR.src, ss.SpinQuantumNumber.HALF, device=ss.Device.CPU)
+        ss.Simulator(F.LARMOR.src, ss.SpinQuantumNumber.HALF, device=ss.Device.CUDA)
 
 
 def test_exponentiation_defaults_and_fallback(capsys):
@@ -358,12 +358,33 @@ def test_phase_plan_of_the_spin_one_workload():
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
-    """Without the built library every entry point raises; nothing falls back
-    to a CPU path."""
+    """Without the built library every ROCm entry point raises; a Device.ROCM
+    simulator never falls back to the CPU devices."""
     monkeypatch.setattr(_native, "LIB_PATH", str(tmp_path / "libspinsim_hip.so"))
     monkeypatch.setattr(_native, "_lib", None)
     with pytest.raises(_native.NativeError, match="no CPU fallback"):
         _native.lib()
     with pytest.raises(_native.NativeError):
         ss.Simulator(F.LARMOR.src + "\n// variant not in the code cache\n",
-                     ss.SpinQuantumNumber.HALF)
+                     ss.SpinQuantumNumber.HALF, device=ss.Device.ROCM)
+
+
+def test_fp32_bulk_override_warns_and_is_recorded():
+    """The float32-bulk modes replace the exponentiation method by a residual
+    series: an explicit exponentiation_method or number_of_squares warns, and
+    Simulator.step_exponential records what computes the step operators."""
+    import warnings
+
+    one = ss.SpinQuantumNumber.ONE
+    with pytest.warns(UserWarning, match="not used"):
+        s = ss.Simulator(F.RABI_ONE.src, one, device=ss.Device.CPU, precision="fp32_bulk",
+                         exponentiation_method=ss.ExponentiationMethod.LIE_TROTTER)
+    assert s.step_exponential == "residual series, float32"
+    with pytest.warns(UserWarning, match="number_of_squares=12"):
+        ss.Simulator(F.RABI_ONE.src, one, device=ss.Device.CPU, precision="fp64_series",
+                     number_of_squares=12)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        s = ss.Simulator(F.RABI_ONE.src, one, device=ss.Device.CPU, precision="fp32_bulk")
+        assert ss.Simulator(F.RABI_ONE.src, one, device=ss.Device.CPU).step_exponential == \
+            "lie_trotter"
