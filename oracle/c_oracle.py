@@ -5,9 +5,11 @@ TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline); only
 leg may use it.  See ``oracle/oracle_core.c`` for what is restated and where.
 
 One shared library is compiled per (field source, spin, method, exponential,
-frame, squares) with ``gcc -std=c11 -O3 -fopenmp`` into ``oracle/_build/``
-(``-march=native`` only when built on the machine that runs it: the cache
-key includes the host CPU model).
+frame, squares) with ``gcc -std=c11 -O3 -fopenmp`` into a cache directory
+outside the source tree (``$SPINSIM_ORACLE_BUILD``, default
+``~/.cache/spinsim_amd/oracle``), so the libraries a process maps from the
+tree are the product's own; ``-march=native`` only when built on the machine
+that runs it (the cache key includes the host CPU model).
 """
 from __future__ import annotations
 
@@ -22,7 +24,8 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_BUILD = os.path.join(_HERE, "_build")
+_BUILD = os.environ.get("SPINSIM_ORACLE_BUILD") or os.path.join(
+    os.path.expanduser("~"), ".cache", "spinsim_amd", "oracle")
 _LOCK = threading.Lock()
 _CACHE: dict = {}
 
