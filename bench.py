@@ -11,17 +11,25 @@ is one evaluate() pass over inputs resident in HBM: the time-evolution kernel,
 the state prefix product with the Bloch vectors fused into its last pass
 (outputs stay on the device).
 
-Also reported (``variants``): the same workload with float32-bulk step
-operators; the batched ensemble of 4096 spin-half Rabi simulations with
-uneven durations (1-50 ms, 50 ns steps, 1 us outputs; float32-bulk; one
-persistent launch + per-simulation state scans replayed as a HIP graph), each
-with its state error against the float64 oracle; and the differentiation
-workload (1024 spin-one pulses, forward + adjoint gradient, float64 and
-float32-bulk forward), with its gradient error against central differences;
-the long noisy spin-half run (1e7 steps); and the spin-half Larmor precession
-(1e6 steps, analytic exponentials).  The batched, long noisy and Larmor
-variants also time the oracle's C restatement on the host cores over the
-oracle runs they check against (``cpu_steps_per_s``, with the sample named).
+Also reported (``variants``): the same workload in float32-bulk, in the
+float64 run of the float32-bulk series (the same algorithm in float64: the
+ratio of the two is what float32 itself buys) and in fp32 (float32-bulk plus
+the field function's sin/cos/exp in float32 on float64-reduced arguments);
+the 4096-simulation spin-half Rabi ensemble with uneven durations (1-50 ms,
+50 ns steps, 1 us outputs) on identical inputs through the batched
+float32-bulk, fp32 and float64 graphs (one persistent launch + batch state
+scan, HIP-graph replay) and through a loop of one plain float64 evaluate per
+simulation (the reference's way through a sweep), each checked on 64 sampled
+simulations against the float64 oracle, with the batched/plain ratios; the
+differentiation workload (1024 spin-one pulses, forward + adjoint gradient,
+float64 and float32-bulk forward, and the deterministic gradient reduction),
+with its gradient error against central differences; the long noisy spin-half
+run (1e7 steps; float64, float32-bulk, fp32); and the spin-half Larmor
+precession (1e6 steps, analytic exponentials).  The CPU baselines (the
+oracle's C restatement) run on every host core this process may use
+(reported with the CPU model), timed over the oracle runs they check
+against (``cpu_steps_per_s``, with the sample named).  ``value`` is withheld
+(null, exit status 1) if the headline's state error exceeds 1e-6.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -445,6 +453,7 @@ def ensemble_same_inputs(ss, torch, steps, cpu_cores, check=64):
     psi0 = ss.SpinQuantumNumber.HALF.plus_z
     refs, cpu_steps, cpu_s = ensemble_oracle_states(sweep, dur, sims, psi0, cpu_cores)
     out = {"batched_fp32_bulk": batched_workload(ss, torch, "fp32_bulk", sweep, dur, refs, steps, 1),
+           "batched_fp32": batched_workload(ss, torch, "fp32", sweep, dur, refs, steps, 1),
            "batched_fp64": batched_workload(ss, torch, "fp64", sweep, dur, refs, steps, 1),
            "plain_fp64_loop": plain_loop_workload(ss, torch, sweep, dur, refs)}
     plain = out["plain_fp64_loop"]["steps_per_s"]
@@ -452,6 +461,9 @@ def ensemble_same_inputs(ss, torch, steps, cpu_cores, check=64):
     out["ratio_batched_fp64_over_plain_fp64"] = out["batched_fp64"]["steps_per_s"] / plain
     out["ratio_fp32_bulk_over_fp64_batched"] = (out["batched_fp32_bulk"]["steps_per_s"]
                                                 / out["batched_fp64"]["steps_per_s"])
+    out["ratio_batched_fp32_over_plain_fp64"] = out["batched_fp32"]["steps_per_s"] / plain
+    out["ratio_fp32_over_fp32_bulk_batched"] = (out["batched_fp32"]["steps_per_s"]
+                                                / out["batched_fp32_bulk"]["steps_per_s"])
     out["target_2x_met"] = bool(out["ratio_batched_fp32_bulk_over_plain_fp64"] >= 2.0 and
                                 out["batched_fp32_bulk"]["state_err_vs_oracle_sampled"] <= 1e-6)
     if cpu_cores:
@@ -509,7 +521,7 @@ def long_noisy_workload(ss, torch, reps=3, cpu_cores=None):
     field = make_long_noisy_field(sampler)
     psi0 = ss.SpinQuantumNumber.HALF.plus_z
     out, states = {}, {}
-    for prec in ("fp64", "fp32_bulk"):
+    for prec in ("fp64", "fp32_bulk", "fp32"):
         sim = ss.Simulator(field, ss.SpinQuantumNumber.HALF, precision=prec)
         res = sim.evaluate(0.0, 0.1, 10e-9, 1e-6, psi0, sweep, output="torch",
                            lanes_per_step="auto")
@@ -526,8 +538,8 @@ def long_noisy_workload(ss, torch, reps=3, cpu_cores=None):
         states[prec] = res.state.cpu().numpy()
         out[prec] = {"steps_per_s": 1e7 / el, "ms_per_evaluate": 1e3 * el,
                      "unitarity_drift": drift}
-    out["fp32_bulk"]["state_err_vs_fp64"] = float(np.abs(states["fp32_bulk"] -
-                                                         states["fp64"]).max())
+    for prec in ("fp32_bulk", "fp32"):
+        out[prec]["state_err_vs_fp64"] = float(np.abs(states[prec] - states["fp64"]).max())
     src, _, _ = ss.transpile(field)
     co = COracle(src, 2, sampler_names=("noise_table",))
     co.set_samplers([(grid, noise, 3, True)])
@@ -535,7 +547,7 @@ def long_noisy_workload(ss, torch, reps=3, cpu_cores=None):
     _, u = co.time_evolution(sweep, 0.0, 1e-8, 1e-6, 100, 100000, threads=cpu_cores)
     ref = co.get_state(psi0, u)
     cpu_s = time.perf_counter() - c0
-    for prec in ("fp64", "fp32_bulk"):
+    for prec in ("fp64", "fp32_bulk", "fp32"):
         out[prec]["state_err_vs_oracle"] = float(np.abs(states[prec] - ref).max())
     if cpu_cores:
         out["cpu_steps_per_s"] = 1e7 / cpu_s
@@ -577,9 +589,9 @@ def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_co
                                 precision=precision)
     amps = torch.tensor(amp, device="cuda")
 
-    def run():
+    def run(deterministic=False):
         a = amps.clone().requires_grad_(True)
-        psi = cs.final_state(a, 0.0, t_end, 20e-9, 1e-6, psi0)
+        psi = cs.final_state(a, 0.0, t_end, 20e-9, 1e-6, psi0, deterministic=deterministic)
         (1 - psi[:, -1].abs() ** 2).sum().backward()
         return a.grad, psi
 
@@ -590,6 +602,19 @@ def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_co
         grad, psi_t = run()
     torch.cuda.synchronize()
     elapsed = (time.perf_counter() - t) / reps
+    # the deterministic gradient reduction: its cost, its agreement with the
+    # atomic sums, and its repeatability
+    det = {}
+    if precision == "fp64":
+        g_det, _ = run(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            g_det2, _ = run(True)
+        torch.cuda.synchronize()
+        det = {"deterministic_forward_backward_ms": 1e3 * (time.perf_counter() - t) / reps,
+               "deterministic_vs_atomic_rel": float((g_det - grad).abs().max() / grad.abs().max()),
+               "deterministic_repeat_identical": bool(torch.equal(g_det, g_det2))}
     psi_t = psi_t.detach().cpu().numpy()
     co = COracle(CONTROLLED_ONE, 3, sampler_names=("ctl_x", "ctl_y"))
     co.set_samplers([(tk, amp[:, 0], 3, True), (tk, amp[:, 1], 3, True)])
@@ -633,7 +658,7 @@ def adjoint_workload(ss, torch, precision, n_sims=1024, reps=3, fd_sims=8, fd_co
             "fine_steps_per_pass": steps, "simulations": n_sims, "control_samples": 2 * m,
             "grad_rel_err_vs_central_fd": rel, "fd_coordinates": len(coords),
             "final_state_err_vs_oracle_sampled": psi_err, "sampled_simulations": check,
-            "precision": precision, **cpu}
+            "precision": precision, **det, **cpu}
 
 
 def fma_probe(torch, threads=256 * 4 * 8 * 64, iters=8192, reps=3):
@@ -707,7 +732,7 @@ def main():
         # series in place of the 24 Lie-Trotter squares) carried out in float64: the
         # ratio of the two is what float32 itself buys, the ratio of the latter to
         # the headline what the algorithm buys
-        for prec in ("fp32_bulk", "fp64_series"):
+        for prec in ("fp32_bulk", "fp64_series", "fp32"):
             v, other_states[prec], _ = single_workload(ss, torch, prec, args.steps, args.warmup,
                                                        lanes)
             variants[prec] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "kernel_ms",

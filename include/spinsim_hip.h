@@ -44,8 +44,11 @@ typedef void *ss_stream; /* hipStream_t */
 /* One interpolation table of generate_interpolation_sampler
  * (spinsim/__init__.py:1941-2142), in device memory: `size` >= 2 ascending
  * times (a table of fewer samples reads as zero, or amplitude[0] when not
- * zero_boundary).  amplitude_stride > 0 selects row `sim` of a (B, size)
- * amplitude array in batched launches. */
+ * zero_boundary).  amplitude_stride > 0 selects row `sim` of a (B, ...)
+ * amplitude array in batched launches.  A multi-axis table holds `axes`
+ * amplitude rows on the one time grid, axis a at amplitude + a*axis_stride
+ * (the reference rejects 2-D amplitudes, :2062-2063); a one-axis table has
+ * axes = 1. */
 typedef struct ss_table {
     const double *time;
     const double *amplitude;
@@ -53,6 +56,9 @@ typedef struct ss_table {
     int64_t amplitude_stride;
     int32_t order;         /* 0 zero-order hold, 1 linear, 3 cubic Hermite */
     int32_t zero_boundary; /* 1: zero outside [time[0], time[-1]] */
+    int32_t axes;          /* components per sample, >= 1 */
+    int32_t reserved;      /* 0 */
+    int64_t axis_stride;   /* doubles between the axes' amplitude rows */
 } ss_table;
 
 /* One differentiated control of the adjoint: field[axis] += scale *
@@ -76,10 +82,14 @@ enum { /* arithmetic of the time-evolution kernel (no reference counterpart) */
     SS_PRECISION_FP32_BULK = 1, /* higher-order residuals of the step operators in
                                    float32; samples, first-order terms, products
                                    across lanes, outputs and scans float64 */
-    SS_PRECISION_FP64_SERIES = 2 /* the float32-bulk operations (residual series in
-                                    place of the exponentiation method) carried
-                                    out in float64: separates that algorithm's
-                                    speed from float32's */
+    SS_PRECISION_FP64_SERIES = 2, /* the float32-bulk operations (residual series in
+                                     place of the exponentiation method) carried
+                                     out in float64: separates that algorithm's
+                                     speed from float32's */
+    SS_PRECISION_FP32 = 3 /* float32 bulk, and the field callback's sin, cos and
+                             exp evaluated in float32 on arguments reduced in
+                             float64 (the callback's other arithmetic, time
+                             and the frame stay float64) */
 };
 enum { /* ExponentiationMethod, :95-129 */
     SS_EXPO_ANALYTIC = 0,
@@ -199,23 +209,37 @@ int ss_batch_final_state(const double *psi0, const double *u, const double *stat
  * amplitudes, for a module whose field reads the controls' tables.  Needs the
  * forward batch's u and state and the costates; recomputes the fine steps.
  * tables: the device ss_table array the module's field reads (n_tables
- * entries); each control reads table controls[c].table < n_tables and owns
- * gradient entries [grad_offset, grad_offset + that table's size) of each row;
- * entries at or beyond grad_stride are dropped.  grad (n_sims, grad_stride)
- * and *status are cleared by the call; status becomes 1 if a step's generator
- * was too large for the derivative series.  (No reference counterpart: the
- * reference has no gradients.) */
+ * entries); each control reads table controls[c].table < n_tables (its first
+ * axis) and owns gradient entries [grad_offset, grad_offset + that table's
+ * size) of each row; entries at or beyond grad_stride are dropped.  grad
+ * (n_sims, grad_stride) and *status are cleared by the call; status bit 1 is
+ * set if a step's generator was too large for the derivative series.
+ * workspace NULL: contributions are summed with float64 atomics, so the
+ * rounding of a gradient entry can differ from run to run (sums of the same
+ * terms in another order).  workspace of ss_batch_adjoint_workspace(total,
+ * n_controls) bytes: deterministic -- each coarse row stores its 8-knot
+ * window per control and a second pass sums every entry's terms in row
+ * order, so repeated calls give identical bits; a row whose samples reach
+ * beyond its window (a coarse step spanning more than about five knot
+ * intervals) sets status bit 2 and its gradient is not complete.  (No
+ * reference counterpart: the reference has no gradients.) */
 int ss_batch_adjoint(const ss_module *m, const double *sweep, int64_t sweep_stride, int64_t n_sims,
                      const int64_t *offsets, const double *t0, const double *dt,
                      const double *dt_out, const int32_t *n_fine, int64_t total, const double *u,
                      const double *state, const double *costate, const ss_table *tables,
                      int n_tables, const ss_control_desc *controls, int n_controls, double *grad,
-                     int64_t grad_stride, int32_t *status, ss_stream stream);
+                     int64_t grad_stride, int32_t *status, void *workspace,
+                     size_t workspace_bytes, ss_stream stream);
+/* Bytes of the deterministic workspace of ss_batch_adjoint (0 for invalid
+ * arguments).  (No reference counterpart.) */
+size_t ss_batch_adjoint_workspace(int64_t total, int n_controls);
 
 /* Evaluate interpolation table k (row `sim` when batched) at n device
- * times: the sampler of generate_interpolation_sampler, :2065-2142. */
-int ss_interp_eval(const ss_table *tables, int k, int sim, const double *t, double *out, int64_t n,
-                   ss_stream stream);
+ * times: the sampler of generate_interpolation_sampler, :2065-2142.  out
+ * receives `axes` components per time, (n, axes) row-major (axes 1 to the
+ * table's axes; each time's components come from one index search). */
+int ss_interp_eval(const ss_table *tables, int k, int sim, int axes, const double *t, double *out,
+                   int64_t n, ss_stream stream);
 
 /* Bloch vector of each of n states (dim complex each): spin is (n, 3). */
 int ss_spin(const double *state, double *spin, int64_t n, int dim, ss_stream stream);

@@ -50,21 +50,30 @@ typedef struct {
     int64_t amplitude_stride;
     int order;
     int zero_boundary;
+    int axes;
+    int reserved;
+    int64_t axis_stride;
 } ss_table;
 typedef struct {
     const ss_table *tables;
     int sim;
 } ss_ctx;
 double ss_interp_sample(const ss_ctx *ctx, int k, double t);
+void ss_interp_sample_axes(const ss_ctx *ctx, int k, double t, double *out, int n);
+double ss_interp_sample_axis(const ss_ctx *ctx, int k, int a, double t);
 #define SS_CONTEXT const ss_ctx *ss_ctx_ref
 #define SS_SAMPLE(k, t) ss_interp_sample(ss_ctx_ref, (k), (t))
+#define SS_SAMPLE_AXES(k, t, out, n) ss_interp_sample_axes(ss_ctx_ref, (k), (t), (out), (n))
+#define SS_SAMPLE_AXIS(k, a, t) ss_interp_sample_axis(ss_ctx_ref, (k), (a), (t))
 """
 
 
 class SsTable(ctypes.Structure):
     _fields_ = [("time", ctypes.c_void_p), ("amplitude", ctypes.c_void_p),
                 ("size", ctypes.c_int64), ("amplitude_stride", ctypes.c_int64),
-                ("order", ctypes.c_int), ("zero_boundary", ctypes.c_int)]
+                ("order", ctypes.c_int), ("zero_boundary", ctypes.c_int),
+                ("axes", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("axis_stride", ctypes.c_int64)]
 
 
 def _cpu_model():
@@ -121,21 +130,21 @@ class COracle:
         self._keep = []
 
     def set_samplers(self, samplers):
-        """samplers: list of (time, amplitude (M,) or (B, M), order, zero_boundary)."""
-        arr = (SsTable * max(len(samplers), 1))()
-        keep = []
-        for k, (time, amp, order, zero_boundary) in enumerate(samplers):
-            t = np.ascontiguousarray(time, np.float64)
-            a = np.ascontiguousarray(amp, np.float64)
-            keep += [t, a]
-            arr[k].time = t.ctypes.data
-            arr[k].amplitude = a.ctypes.data
-            arr[k].size = t.size
-            arr[k].amplitude_stride = a.shape[-1] if a.ndim == 2 else 0
-            arr[k].order = int(order)
-            arr[k].zero_boundary = int(bool(zero_boundary))
-        self._tables = arr
-        self._keep = keep
+        """samplers: list of (time, amplitude, order, zero_boundary[, multi_axis]).
+        amplitude (M,): one component; (B, M): one row per simulation; with
+        multi_axis=True, (A, M): A components, or (B, A, M) per simulation."""
+        self._tables, self._keep = tables_of(samplers)
+
+    def interp_eval(self, k, t, sim=0, axes=1):
+        """Sampler k (row `sim`) at times t: (n, axes) values (n values when axes = 1)."""
+        self.lib.oracle_interp_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_int64]
+        t = np.ascontiguousarray(np.atleast_1d(t), np.float64)
+        out = np.zeros((t.size, axes), np.float64)
+        self.lib.oracle_interp_eval(ctypes.addressof(self._tables), int(k), int(sim), int(axes),
+                                    t.ctypes.data, out.ctypes.data, t.size)
+        return out[:, 0] if axes == 1 else out
 
     def time_evolution(self, sweep, t0, dt, dt_out, n_fine, n_coarse, i_begin=0, i_end=None,
                        sim=0, threads=None, out=None):
@@ -168,6 +177,36 @@ class COracle:
         self.lib.oracle_get_spin(state.ctypes.data, spin.ctypes.data, state.shape[0],
                                  int(threads))
         return spin
+
+
+def tables_of(samplers):
+    """(ctypes ss_table array, the arrays its records point at) for samplers
+    given as (time, amplitude, order, zero_boundary[, multi_axis])."""
+    arr = (SsTable * max(len(samplers), 1))()
+    keep = []
+    for k, entry in enumerate(samplers):
+        time, amp, order, zero_boundary = entry[:4]
+        multi = bool(entry[4]) if len(entry) > 4 else False
+        t = np.ascontiguousarray(time, np.float64)
+        a = np.ascontiguousarray(amp, np.float64)
+        keep += [t, a]
+        m = t.size
+        if multi:
+            axes = a.shape[-2]
+            rows_stride = axes * m if a.ndim == 3 else 0
+        else:
+            axes = 1
+            rows_stride = m if a.ndim == 2 else 0
+        arr[k].time = t.ctypes.data
+        arr[k].amplitude = a.ctypes.data
+        arr[k].size = m
+        arr[k].amplitude_stride = rows_stride
+        arr[k].order = int(order)
+        arr[k].zero_boundary = int(bool(zero_boundary))
+        arr[k].axes = axes
+        arr[k].reserved = 0
+        arr[k].axis_stride = m
+    return arr, keep
 
 
 def _load(src, flags, key):

@@ -64,11 +64,16 @@ static int ss_find_time_index(const ss_table *tb, double t, double *param)
     return -1;
 }
 
-double ss_interp_sample(const ss_ctx *ctx, int k, double t)
+/* The sampler of :2065-2142 on amplitude row `axis` of table k (axis 0 for a
+ * one-axis table).  The multi-axis form has no reference counterpart (:2062-2063
+ * raises for 2-D amplitudes): each component is the one-axis sampler of its
+ * row, with an index search of its own. */
+static double interp_row(const ss_ctx *ctx, int k, int64_t axis, double t)
 {
     const ss_table *tb = &ctx->tables[k];
     const double *time = tb->time;
-    const double *amp = tb->amplitude + (int64_t)ctx->sim * tb->amplitude_stride;
+    const double *amp = tb->amplitude + (int64_t)ctx->sim * tb->amplitude_stride
+                        + axis * tb->axis_stride;
     int64_t size = tb->size;
     double out = 0;
     if (!tb->zero_boundary) {
@@ -102,6 +107,30 @@ double ss_interp_sample(const ss_ctx *ctx, int k, double t)
         }
     }
     return out;
+}
+
+double ss_interp_sample(const ss_ctx *ctx, int k, double t) { return interp_row(ctx, k, 0, t); }
+
+void ss_interp_sample_axes(const ss_ctx *ctx, int k, double t, double *out, int n)
+{
+    const int axes = ctx->tables[k].axes;
+    for (int a = 0; a < n && a < axes; ++a) out[a] = interp_row(ctx, k, a, t);
+}
+
+double ss_interp_sample_axis(const ss_ctx *ctx, int k, int a, double t)
+{
+    return (a >= 0 && a < ctx->tables[k].axes) ? interp_row(ctx, k, a, t) : 0.0;
+}
+
+/* The sampler of table k (row `sim`) at n times, `axes` components each. */
+int oracle_interp_eval(const ss_table *tables, int k, int sim, int axes, const double *t, double *out,
+                       int64_t n)
+{
+    ss_ctx ctx;
+    ctx.tables = tables;
+    ctx.sim = sim;
+    for (int64_t i = 0; i < n; ++i) ss_interp_sample_axes(&ctx, k, t[i], out + i * axes, axes);
+    return 0;
 }
 
 /* ---- small complex helpers, :1552-1615 / :1679-1761 -------------------- */

@@ -53,15 +53,16 @@ SIGNATURES = {
     "ss_batch_final_state": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp]),
     "ss_batch_adjoint": (c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64,
                                  c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_i64, c_vp,
-                                 c_vp]),
-    "ss_interp_eval": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_i64, c_vp]),
+                                 c_vp, ctypes.c_size_t, c_vp]),
+    "ss_batch_adjoint_workspace": (ctypes.c_size_t, [c_i64, c_int]),
+    "ss_interp_eval": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_i64, c_vp]),
     "ss_spin": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
     "ss_fma_probe": (c_int, [c_vp, c_i64, c_int, c_vp]),
 }
 
 
 # Version of the C ABI these signatures describe (ss_abi_version()).
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 
 class NativeError(RuntimeError):

@@ -106,12 +106,17 @@ class ControlledSimulator:
         self.grad_stride = off
 
     def final_state(self, amplitudes, time_start, time_end, time_step_integration,
-                    time_step_output, state_init, sweep_parameters=None):
+                    time_step_output, state_init, sweep_parameters=None, *,
+                    deterministic=False):
         """psi(T) (B, d) complex128 of every simulation; differentiable in ``amplitudes``.
 
         ``amplitudes``: float64 device tensor (B, K, M_k) -- one row of control
         samples per simulation (all controls must share M when K > 1, or pass
-        a list of (B, M_k) tensors).
+        a list of (B, M_k) tensors).  ``deterministic=True`` sums each gradient
+        entry's terms in a fixed order (bit-identical from run to run; needs a
+        workspace of 72 B per coarse row and control, and coarse steps that
+        span at most about five knot intervals); by default they are summed
+        with float64 atomics, whose order -- and so the last bits -- can vary.
         """
         import torch
 
@@ -126,13 +131,14 @@ class ControlledSimulator:
             raise ValueError(f"expected {self.grad_stride} control samples per simulation, "
                              f"got {amps.shape[1]}")
         return _FinalState.apply(amps, self, (time_start, time_end, time_step_integration,
-                                              time_step_output, state_init, sweep_parameters))
+                                              time_step_output, state_init, sweep_parameters,
+                                              bool(deterministic)))
 
     # ---- used by the autograd function -----------------------------------------
     def _forward(self, amps, args):
         import torch
 
-        t0, t1, dti, dto, psi0, sweep = args
+        t0, t1, dti, dto, psi0, sweep, _ = args
         b = amps.shape[0]
         amps = amps.detach().contiguous()
         for c, off in zip(self.controls, self.grad_offsets):
@@ -152,7 +158,7 @@ class ControlledSimulator:
             "ss_batch_final_state")
         return out, plan, amps
 
-    def _backward(self, plan, amps, grad_psi):
+    def _backward(self, plan, amps, grad_psi, deterministic=False):
         import torch
 
         b, d = plan.n_sims, self.dimension
@@ -177,6 +183,10 @@ class ControlledSimulator:
             descs[k].grad_offset = off
         grad = torch.empty((b, self.grad_stride), dtype=torch.float64, device=dev)
         status = torch.empty(1, dtype=torch.int32, device=dev)
+        ws, ws_bytes = None, 0
+        if deterministic:
+            ws_bytes = int(L.ss_batch_adjoint_workspace(plan.total, len(self.controls)))
+            ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         mod = self.sim._module(dev)
         _native.check(L.ss_batch_adjoint(
             mod.handle, plan.sweep.data_ptr(), plan.sweep.shape[1], b, plan.offsets.data_ptr(),
@@ -184,11 +194,16 @@ class ControlledSimulator:
             plan.n_fine.data_ptr(), plan.total, plan.u.data_ptr(), plan.state.data_ptr(),
             costate.data_ptr(), self.sim._table_ptr(dev), len(self.sim.field.samplers),
             ctypes.addressof(descs), len(self.controls), grad.data_ptr(), self.grad_stride,
-            status.data_ptr(), stream),
+            status.data_ptr(), None if ws is None else ws.data_ptr(), ws_bytes, stream),
             "ss_batch_adjoint")
-        if int(status.item()) != 0:
+        flags = int(status.item())
+        if flags & 1:
             raise RuntimeError("ss_batch_adjoint: a step's rotation angle exceeds the derivative "
                                "series range; use a smaller time_step_integration")
+        if flags & 2:
+            raise RuntimeError("ss_batch_adjoint: a coarse step's samples reach beyond its "
+                               "8-knot gradient window, which the deterministic mode needs; "
+                               "use a shorter time_step_output or deterministic=False")
         return grad
 
 
@@ -199,12 +214,12 @@ def _final_state_function():
         @staticmethod
         def forward(ctx, amps, owner, args):
             out, plan, amps_c = owner._forward(amps, args)
-            ctx.owner, ctx.plan, ctx.amps = owner, plan, amps_c
+            ctx.owner, ctx.plan, ctx.amps, ctx.deterministic = owner, plan, amps_c, args[-1]
             return out
 
         @staticmethod
         def backward(ctx, grad_out):
-            grad = ctx.owner._backward(ctx.plan, ctx.amps, grad_out)
+            grad = ctx.owner._backward(ctx.plan, ctx.amps, grad_out, ctx.deterministic)
             return grad, None, None
 
     return _Fn

@@ -85,7 +85,11 @@ struct ss_adjoint_args {
     const double* state;         // (total, d)
     const double* costate;       // (total, d)
     double* grad;                // (B, grad_stride), accumulated with atomics
-    int* status;                 // set to 1 if a step is too large for the series
+    int* status;                 // bit 1: a step too large for the series;
+                                 // bit 2: a sample outside the window (deterministic mode)
+    double* partial;             // deterministic mode: (total, n_controls, SS_ADJ_WINDOW)
+                                 // window contents per row, or null (atomics)
+    long long* window_base;      // deterministic mode: (total, n_controls) window starts
     long long total;
     long long n_sims;
     long long sweep_stride;

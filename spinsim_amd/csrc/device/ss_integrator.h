@@ -7,8 +7,10 @@
 //   SS_ROT 0|1          rotating frame
 //   SS_HYPER h          Lie-Trotter hypercubes (2h residual squarings)
 //   SS_CTX 0|1          callback takes the sampler context argument
-//   SS_PREC 0|1|2       plain float64 / float32 bulk / the float32-bulk series in
-//                       float64 (a precision ablation of the same arithmetic)
+//   SS_PREC 0|1|2|3     plain float64 / float32 bulk / the float32-bulk series in
+//                       float64 (a precision ablation of the same arithmetic) /
+//                       float32 bulk with the callback's sin, cos and exp in
+//                       float32 (arguments reduced in float64; see ss_prelude.h)
 //   SS_HOST             defined by the host (CPU device) build: the integrator
 //                       functions only, without the GPU kernels
 //
@@ -458,7 +460,7 @@ static __device__ __forceinline__ void ss_append(const double g[SS_FIELD_LEN], s
 // of residuals are float32.  A chunk's product is I + A(sum g) + b with b
 // float32 of size |A|^2, so float32 rounding acts on second-order terms only.
 // SS_PREC 2 runs the same operations with ss_bulk = double.
-#if SS_PREC == 1
+#if SS_PREC == 1 || SS_PREC == 3
 typedef float ss_bulk;
 #define SS_BULK_FMA fmaf
 #else
@@ -1218,7 +1220,7 @@ static __device__ __forceinline__ void ss_adjoint_exp(const double g[SS_FIELD_LE
 #endif
     int split = 0;
     while (nu * ldexp(1.0, -split) > SS_ADJ_NU && split < SS_ADJ_MAX_SPLIT) ++split;
-    if (nu * ldexp(1.0, -split) > SS_ADJ_NU) *status = 1;
+    if (nu * ldexp(1.0, -split) > SS_ADJ_NU) atomicOr(status, 1);
     const double scale = ldexp(1.0, -split);
     const ss_gen64 a = ss_generator64(g, scale);
     const int pieces = 1 << split;
@@ -1292,7 +1294,9 @@ static __device__ __forceinline__ void ss_interp_weights(const ss_table* tb, dou
     w[3] = h11 * r_2;
 }
 
-// dL/d(lab-frame field) at one sample -> control gradients (LDS window or atomics).
+// dL/d(lab-frame field) at one sample -> control gradients (LDS window or
+// atomics; in the deterministic mode a sample outside the window sets status
+// bit 2 instead, and the call fails).
 static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& a, const ss_ctx* ctx, double t,
                                                          const double dfield[4], double* win,
                                                          const long long base[SS_MAX_CONTROLS], double* grad_row)
@@ -1312,6 +1316,8 @@ static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& 
             const long long slot = idx - base[c];
             if (slot >= 0 && slot < SS_ADJ_WINDOW)
                 win[(c * SS_ADJ_WINDOW + slot) * SS_ADJ_BLOCK] += v * w[k];
+            else if (a.partial)
+                atomicOr(a.status, 2);
             else if (ct.grad_offset + idx < a.grad_stride)
                 atomicAdd(grad_row + ct.grad_offset + idx, v * w[k]);
         }
@@ -1375,9 +1381,13 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
         const int ax = a.controls[c].axis;
         mask |= (ax <= 1 && SS_ROT) ? 3u : (1u << ax);
         z_control = z_control || ax == 2;
+        // the window starts one knot before the interval holding tc; past the
+        // last knot it covers the last knots (so window starts never decrease
+        // along a simulation's rows)
+        const ss_table* tb = &ctx.tables[a.controls[c].table];
         double p;
-        const long long i0 = ss_find_time_index(&ctx.tables[a.controls[c].table], tc, &p);
-        base[c] = (i0 < 0 ? 0 : i0) - 1;
+        const long long i0 = ss_find_time_index(tb, tc, &p);
+        base[c] = (i0 >= 0 ? i0 : (tb->size > 1 && tc > tb->time[tb->size - 1] ? tb->size - 1 : 0)) - 1;
     }
 #if SS_ROT
     // the frame term of a z control needs dL/dg of x, y and z
@@ -1499,6 +1509,17 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
         ss_scatter_sample(a, &ctx, tc + dt_out / 2, dmid, win, base, grad_row);
     }
 #endif
+    if (a.partial) {
+        // deterministic mode: the window goes to this row's record; a second
+        // pass sums the records of each knot in row order
+        for (int c = 0; c < a.n_controls; ++c) {
+            double* rec = a.partial + (row * a.n_controls + c) * SS_ADJ_WINDOW;
+#pragma unroll
+            for (int s = 0; s < SS_ADJ_WINDOW; ++s) rec[s] = win[(c * SS_ADJ_WINDOW + s) * SS_ADJ_BLOCK];
+            a.window_base[row * a.n_controls + c] = base[c];
+        }
+        return;
+    }
     for (int c = 0; c < a.n_controls; ++c) {
         const long long size = ctx.tables[a.controls[c].table].size;
 #pragma unroll

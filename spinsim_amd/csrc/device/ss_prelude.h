@@ -12,7 +12,9 @@
 // field[0..2] are the x, y, z angular frequencies (rad/s) and field[3] the
 // spin-one quadratic shift.  Entries the callback leaves untouched are zero.
 // Interpolation samplers are read with SS_SAMPLE(k, t) or by the name they
-// were registered under (a macro defined after this prelude).
+// were registered under (a macro defined after this prelude); a multi-axis
+// sampler with SS_SAMPLE_AXES(k, t, out, n) (all n components from one index
+// search) or SS_SAMPLE_AXIS(k, a, t) (component a).
 #pragma once
 
 #define SS_PI 3.14159265358979323846
@@ -24,7 +26,8 @@
 // One interpolation table (generate_interpolation_sampler's frozen
 // (time, amplitude) pair, spinsim/__init__.py:1941), held in device memory so
 // new samples need no recompilation.  amplitude_stride > 0 gives one row per
-// simulation of a batch.  Layout shared with include/spinsim_hip.h.
+// simulation of a batch; `axes` components share the time grid, component a
+// at amplitude + a * axis_stride.  Layout shared with include/spinsim_hip.h.
 struct ss_table {
     const double* time;
     const double* amplitude;
@@ -32,6 +35,9 @@ struct ss_table {
     long long amplitude_stride;
     int order;
     int zero_boundary;
+    int axes;
+    int reserved;
+    long long axis_stride;
 };
 
 // The interval a table was last sampled in, as the cubic
@@ -58,8 +64,211 @@ struct ss_ctx {
     }
 };
 
+// ---- float32 transcendentals for field callbacks (precision "fp32") --------
+// In that mode the callback's sin(x), cos(x) and exp(x) resolve to these.
+// Arguments are reduced in float64 by Cody-Waite steps (the constant split in
+// two, fused multiply-adds), so no phase is rounded to float32; float32 then
+// carries only the higher-order terms:
+//   sin, cos: x = k h + b, h = 2 pi / 256, |b| <= h / 2, a = k h from a table
+//     of float64 cos a, sin a (cos and sin of j 2 pi / 256, correctly rounded):
+//       cos x = cos a (1 + cm) - sin a b (1 + sm),
+//       sin x = sin a (1 + cm) + cos a b (1 + sm),
+//     cm = cos b - 1 = -b^2/2 + b^4/24, sm = sin b / b - 1 = -b^2/6 + b^4/120
+//     in float32 (|cm| < 7.6e-5, |sm| < 2.6e-5: their rounding is about 1e-11
+//     absolute in the result; truncation below 1e-17).
+//   exp: x = n ln2 + r, |r| <= ln2 / 2, r = rh + rl with rh = float32(r):
+//     exp x = 2^n expf(rh) (1 + rl), about one float32 ulp (6e-8) relative.
+// Arguments outside the reductions' range (and NaN) take the float64 functions.
+#define SS_TRIG_H_HI 0.02454369260617026
+#define SS_TRIG_H_LO 9.567553118338697e-19
+#define SS_TRIG_INV_H 40.74366543152521
+#define SS_LN2_HI 0.6931471805599453
+#define SS_LN2_LO 2.3190468138462996e-17
+#define SS_INV_LN2 1.4426950408889634
+
+// cos(2 pi j / 256), sin(2 pi j / 256), j = 0 .. 255
+__constant__ static const double ss_trig_table[512] = {
+    1.0, 0.0, 0.9996988186962042, 0.024541228522912288,
+    0.9987954562051724, 0.049067674327418015, 0.9972904566786902, 0.07356456359966743,
+    0.9951847266721969, 0.0980171403295606, 0.99247953459871, 0.1224106751992162,
+    0.989176509964781, 0.14673047445536175, 0.9852776423889412, 0.17096188876030122,
+    0.9807852804032304, 0.19509032201612828, 0.9757021300385286, 0.2191012401568698,
+    0.970031253194544, 0.2429801799032639, 0.9637760657954398, 0.26671275747489837,
+    0.9569403357322088, 0.2902846772544624, 0.9495281805930367, 0.31368174039889146,
+    0.9415440651830208, 0.33688985339222005, 0.9329927988347388, 0.35989503653498817,
+    0.9238795325112867, 0.3826834323650898, 0.9142097557035307, 0.40524131400498986,
+    0.9039892931234433, 0.4275550934302821, 0.8932243011955153, 0.4496113296546066,
+    0.881921264348355, 0.47139673682599764, 0.8700869911087115, 0.49289819222978404,
+    0.8577286100002721, 0.5141027441932218, 0.8448535652497071, 0.5349976198870973,
+    0.8314696123025452, 0.5555702330196022, 0.8175848131515837, 0.5758081914178453,
+    0.8032075314806449, 0.5956993044924334, 0.7883464276266062, 0.6152315905806268,
+    0.773010453362737, 0.6343932841636455, 0.7572088465064846, 0.6531728429537768,
+    0.7409511253549591, 0.6715589548470184, 0.7242470829514669, 0.6895405447370669,
+    0.7071067811865476, 0.7071067811865476, 0.6895405447370669, 0.7242470829514669,
+    0.6715589548470184, 0.7409511253549591, 0.6531728429537768, 0.7572088465064846,
+    0.6343932841636455, 0.773010453362737, 0.6152315905806268, 0.7883464276266062,
+    0.5956993044924334, 0.8032075314806449, 0.5758081914178453, 0.8175848131515837,
+    0.5555702330196022, 0.8314696123025452, 0.5349976198870973, 0.8448535652497071,
+    0.5141027441932218, 0.8577286100002721, 0.49289819222978404, 0.8700869911087115,
+    0.47139673682599764, 0.881921264348355, 0.4496113296546066, 0.8932243011955153,
+    0.4275550934302821, 0.9039892931234433, 0.40524131400498986, 0.9142097557035307,
+    0.3826834323650898, 0.9238795325112867, 0.35989503653498817, 0.9329927988347388,
+    0.33688985339222005, 0.9415440651830208, 0.31368174039889146, 0.9495281805930367,
+    0.2902846772544624, 0.9569403357322088, 0.26671275747489837, 0.9637760657954398,
+    0.2429801799032639, 0.970031253194544, 0.2191012401568698, 0.9757021300385286,
+    0.19509032201612828, 0.9807852804032304, 0.17096188876030122, 0.9852776423889412,
+    0.14673047445536175, 0.989176509964781, 0.1224106751992162, 0.99247953459871,
+    0.0980171403295606, 0.9951847266721969, 0.07356456359966743, 0.9972904566786902,
+    0.049067674327418015, 0.9987954562051724, 0.024541228522912288, 0.9996988186962042,
+    2.0670321098263988e-43, 1.0, -0.024541228522912288, 0.9996988186962042,
+    -0.049067674327418015, 0.9987954562051724, -0.07356456359966743, 0.9972904566786902,
+    -0.0980171403295606, 0.9951847266721969, -0.1224106751992162, 0.99247953459871,
+    -0.14673047445536175, 0.989176509964781, -0.17096188876030122, 0.9852776423889412,
+    -0.19509032201612828, 0.9807852804032304, -0.2191012401568698, 0.9757021300385286,
+    -0.2429801799032639, 0.970031253194544, -0.26671275747489837, 0.9637760657954398,
+    -0.2902846772544624, 0.9569403357322088, -0.31368174039889146, 0.9495281805930367,
+    -0.33688985339222005, 0.9415440651830208, -0.35989503653498817, 0.9329927988347388,
+    -0.3826834323650898, 0.9238795325112867, -0.40524131400498986, 0.9142097557035307,
+    -0.4275550934302821, 0.9039892931234433, -0.4496113296546066, 0.8932243011955153,
+    -0.47139673682599764, 0.881921264348355, -0.49289819222978404, 0.8700869911087115,
+    -0.5141027441932218, 0.8577286100002721, -0.5349976198870973, 0.8448535652497071,
+    -0.5555702330196022, 0.8314696123025452, -0.5758081914178453, 0.8175848131515837,
+    -0.5956993044924334, 0.8032075314806449, -0.6152315905806268, 0.7883464276266062,
+    -0.6343932841636455, 0.773010453362737, -0.6531728429537768, 0.7572088465064846,
+    -0.6715589548470184, 0.7409511253549591, -0.6895405447370669, 0.7242470829514669,
+    -0.7071067811865476, 0.7071067811865476, -0.7242470829514669, 0.6895405447370669,
+    -0.7409511253549591, 0.6715589548470184, -0.7572088465064846, 0.6531728429537768,
+    -0.773010453362737, 0.6343932841636455, -0.7883464276266062, 0.6152315905806268,
+    -0.8032075314806449, 0.5956993044924334, -0.8175848131515837, 0.5758081914178453,
+    -0.8314696123025452, 0.5555702330196022, -0.8448535652497071, 0.5349976198870973,
+    -0.8577286100002721, 0.5141027441932218, -0.8700869911087115, 0.49289819222978404,
+    -0.881921264348355, 0.47139673682599764, -0.8932243011955153, 0.4496113296546066,
+    -0.9039892931234433, 0.4275550934302821, -0.9142097557035307, 0.40524131400498986,
+    -0.9238795325112867, 0.3826834323650898, -0.9329927988347388, 0.35989503653498817,
+    -0.9415440651830208, 0.33688985339222005, -0.9495281805930367, 0.31368174039889146,
+    -0.9569403357322088, 0.2902846772544624, -0.9637760657954398, 0.26671275747489837,
+    -0.970031253194544, 0.2429801799032639, -0.9757021300385286, 0.2191012401568698,
+    -0.9807852804032304, 0.19509032201612828, -0.9852776423889412, 0.17096188876030122,
+    -0.989176509964781, 0.14673047445536175, -0.99247953459871, 0.1224106751992162,
+    -0.9951847266721969, 0.0980171403295606, -0.9972904566786902, 0.07356456359966743,
+    -0.9987954562051724, 0.049067674327418015, -0.9996988186962042, 0.024541228522912288,
+    -1.0, 4.1340642196527976e-43, -0.9996988186962042, -0.024541228522912288,
+    -0.9987954562051724, -0.049067674327418015, -0.9972904566786902, -0.07356456359966743,
+    -0.9951847266721969, -0.0980171403295606, -0.99247953459871, -0.1224106751992162,
+    -0.989176509964781, -0.14673047445536175, -0.9852776423889412, -0.17096188876030122,
+    -0.9807852804032304, -0.19509032201612828, -0.9757021300385286, -0.2191012401568698,
+    -0.970031253194544, -0.2429801799032639, -0.9637760657954398, -0.26671275747489837,
+    -0.9569403357322088, -0.2902846772544624, -0.9495281805930367, -0.31368174039889146,
+    -0.9415440651830208, -0.33688985339222005, -0.9329927988347388, -0.35989503653498817,
+    -0.9238795325112867, -0.3826834323650898, -0.9142097557035307, -0.40524131400498986,
+    -0.9039892931234433, -0.4275550934302821, -0.8932243011955153, -0.4496113296546066,
+    -0.881921264348355, -0.47139673682599764, -0.8700869911087115, -0.49289819222978404,
+    -0.8577286100002721, -0.5141027441932218, -0.8448535652497071, -0.5349976198870973,
+    -0.8314696123025452, -0.5555702330196022, -0.8175848131515837, -0.5758081914178453,
+    -0.8032075314806449, -0.5956993044924334, -0.7883464276266062, -0.6152315905806268,
+    -0.773010453362737, -0.6343932841636455, -0.7572088465064846, -0.6531728429537768,
+    -0.7409511253549591, -0.6715589548470184, -0.7242470829514669, -0.6895405447370669,
+    -0.7071067811865476, -0.7071067811865476, -0.6895405447370669, -0.7242470829514669,
+    -0.6715589548470184, -0.7409511253549591, -0.6531728429537768, -0.7572088465064846,
+    -0.6343932841636455, -0.773010453362737, -0.6152315905806268, -0.7883464276266062,
+    -0.5956993044924334, -0.8032075314806449, -0.5758081914178453, -0.8175848131515837,
+    -0.5555702330196022, -0.8314696123025452, -0.5349976198870973, -0.8448535652497071,
+    -0.5141027441932218, -0.8577286100002721, -0.49289819222978404, -0.8700869911087115,
+    -0.47139673682599764, -0.881921264348355, -0.4496113296546066, -0.8932243011955153,
+    -0.4275550934302821, -0.9039892931234433, -0.40524131400498986, -0.9142097557035307,
+    -0.3826834323650898, -0.9238795325112867, -0.35989503653498817, -0.9329927988347388,
+    -0.33688985339222005, -0.9415440651830208, -0.31368174039889146, -0.9495281805930367,
+    -0.2902846772544624, -0.9569403357322088, -0.26671275747489837, -0.9637760657954398,
+    -0.2429801799032639, -0.970031253194544, -0.2191012401568698, -0.9757021300385286,
+    -0.19509032201612828, -0.9807852804032304, -0.17096188876030122, -0.9852776423889412,
+    -0.14673047445536175, -0.989176509964781, -0.1224106751992162, -0.99247953459871,
+    -0.0980171403295606, -0.9951847266721969, -0.07356456359966743, -0.9972904566786902,
+    -0.049067674327418015, -0.9987954562051724, -0.024541228522912288, -0.9996988186962042,
+    2.2338764406549882e-41, -1.0, 0.024541228522912288, -0.9996988186962042,
+    0.049067674327418015, -0.9987954562051724, 0.07356456359966743, -0.9972904566786902,
+    0.0980171403295606, -0.9951847266721969, 0.1224106751992162, -0.99247953459871,
+    0.14673047445536175, -0.989176509964781, 0.17096188876030122, -0.9852776423889412,
+    0.19509032201612828, -0.9807852804032304, 0.2191012401568698, -0.9757021300385286,
+    0.2429801799032639, -0.970031253194544, 0.26671275747489837, -0.9637760657954398,
+    0.2902846772544624, -0.9569403357322088, 0.31368174039889146, -0.9495281805930367,
+    0.33688985339222005, -0.9415440651830208, 0.35989503653498817, -0.9329927988347388,
+    0.3826834323650898, -0.9238795325112867, 0.40524131400498986, -0.9142097557035307,
+    0.4275550934302821, -0.9039892931234433, 0.4496113296546066, -0.8932243011955153,
+    0.47139673682599764, -0.881921264348355, 0.49289819222978404, -0.8700869911087115,
+    0.5141027441932218, -0.8577286100002721, 0.5349976198870973, -0.8448535652497071,
+    0.5555702330196022, -0.8314696123025452, 0.5758081914178453, -0.8175848131515837,
+    0.5956993044924334, -0.8032075314806449, 0.6152315905806268, -0.7883464276266062,
+    0.6343932841636455, -0.773010453362737, 0.6531728429537768, -0.7572088465064846,
+    0.6715589548470184, -0.7409511253549591, 0.6895405447370669, -0.7242470829514669,
+    0.7071067811865476, -0.7071067811865476, 0.7242470829514669, -0.6895405447370669,
+    0.7409511253549591, -0.6715589548470184, 0.7572088465064846, -0.6531728429537768,
+    0.773010453362737, -0.6343932841636455, 0.7883464276266062, -0.6152315905806268,
+    0.8032075314806449, -0.5956993044924334, 0.8175848131515837, -0.5758081914178453,
+    0.8314696123025452, -0.5555702330196022, 0.8448535652497071, -0.5349976198870973,
+    0.8577286100002721, -0.5141027441932218, 0.8700869911087115, -0.49289819222978404,
+    0.881921264348355, -0.47139673682599764, 0.8932243011955153, -0.4496113296546066,
+    0.9039892931234433, -0.4275550934302821, 0.9142097557035307, -0.40524131400498986,
+    0.9238795325112867, -0.3826834323650898, 0.9329927988347388, -0.35989503653498817,
+    0.9415440651830208, -0.33688985339222005, 0.9495281805930367, -0.31368174039889146,
+    0.9569403357322088, -0.2902846772544624, 0.9637760657954398, -0.26671275747489837,
+    0.970031253194544, -0.2429801799032639, 0.9757021300385286, -0.2191012401568698,
+    0.9807852804032304, -0.19509032201612828, 0.9852776423889412, -0.17096188876030122,
+    0.989176509964781, -0.14673047445536175, 0.99247953459871, -0.1224106751992162,
+    0.9951847266721969, -0.0980171403295606, 0.9972904566786902, -0.07356456359966743,
+    0.9987954562051724, -0.049067674327418015, 0.9996988186962042, -0.024541228522912288,
+};
+
+// x = k h + b: the table entry of k and b; false when x is out of range.
+__device__ static inline bool ss_trig_reduce(double x, double* ca, double* sa, double* b)
+{
+    if (!(fabs(x) < 1e15)) return false;
+    const double k = rint(x * SS_TRIG_INV_H);
+    *b = fma(-k, SS_TRIG_H_LO, fma(-k, SS_TRIG_H_HI, x));
+    const int j = (int)((long long)k & 255);
+    *ca = ss_trig_table[2 * j];
+    *sa = ss_trig_table[2 * j + 1];
+    return true;
+}
+
+// cm = cos b - 1 and sm = sin b / b - 1 in float32.
+__device__ static inline void ss_trig_small(double b, double* cm, double* sm)
+{
+    const float bf = (float)b;
+    const float b2 = bf * bf;
+    *cm = (double)(b2 * fmaf(b2, 1.0f / 24.0f, -0.5f));
+    *sm = (double)(b2 * fmaf(b2, 1.0f / 120.0f, -1.0f / 6.0f));
+}
+
+__device__ static inline double ss_cos32(double x)
+{
+    double ca, sa, b, cm, sm;
+    if (!ss_trig_reduce(x, &ca, &sa, &b)) return cos(x);
+    ss_trig_small(b, &cm, &sm);
+    return fma(-sa, fma(b, sm, b), fma(ca, cm, ca));
+}
+
+__device__ static inline double ss_sin32(double x)
+{
+    double ca, sa, b, cm, sm;
+    if (!ss_trig_reduce(x, &ca, &sa, &b)) return sin(x);
+    ss_trig_small(b, &cm, &sm);
+    return fma(ca, fma(b, sm, b), fma(sa, cm, sa));
+}
+
+__device__ static inline double ss_exp32(double x)
+{
+    if (!(fabs(x) < 700.0)) return exp(x);
+    const double n = rint(x * SS_INV_LN2);
+    const double r = fma(-n, SS_LN2_LO, fma(-n, SS_LN2_HI, x));
+    const float rh = (float)r;
+    const double e = (double)expf(rh);
+    return ldexp(fma(e, r - (double)rh, e), (int)n);
+}
+
 #define SS_CONTEXT const ss_ctx* ss_ctx_ref
 #define SS_SAMPLE(k, t) ss_interp_sample(ss_ctx_ref, (k), (t))
+#define SS_SAMPLE_AXES(k, t, out, n) ss_interp_sample_axes(ss_ctx_ref, (k), (t), (out), (n))
+#define SS_SAMPLE_AXIS(k, a, t) ss_interp_sample_axis(ss_ctx_ref, (k), (a), (t))
 
 // find_time_index, spinsim/__init__.py:2026-2061: uniform-grid guess, checked,
 // then bisection capped at `size` iterations; -1 when out of range.
@@ -98,6 +307,27 @@ __device__ static inline long long ss_find_time_index(const ss_table* tb, double
     }
     *param = 0.0;
     return -1;
+}
+
+// Cubic Hermite tangents of interval i scaled by its width (m_left, m_right of
+// :2107-2136: averaged neighbouring secants, the secant itself at the ends,
+// zero next to a repeated knot).
+__device__ static inline void ss_interp_tangents(const double* time, const double* amp, long long size,
+                                                 long long i, double* left, double* right)
+{
+    const double h = time[i + 1] - time[i];
+    const double mid = (time[i] != time[i + 1]) ? (amp[i + 1] - amp[i]) / h : 0.0;
+    double l, r;
+    if (i == 0) l = mid;
+    else if (time[i - 1] != time[i])
+        l = 0.5 * ((amp[i] - amp[i - 1]) / (time[i] - time[i - 1]) + mid);
+    else l = 0.0;
+    if (i + 2 == size) r = mid;
+    else if (time[i + 1] != time[i + 2])
+        r = 0.5 * ((amp[i + 2] - amp[i + 1]) / (time[i + 2] - time[i + 1]) + mid);
+    else r = 0.0;
+    *left = l * h;
+    *right = r * h;
 }
 
 // sampler(), spinsim/__init__.py:2065-2142 (orders 0, 1 and 3).  Linear and
@@ -141,18 +371,8 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
         }
     } else if (i >= 0 && i + 1 < size) {
         const double h = time[i + 1] - time[i];
-        const double mid = (time[i] != time[i + 1]) ? (amp[i + 1] - amp[i]) / h : 0.0;
         double left, right;
-        if (i == 0) left = mid;
-        else if (time[i - 1] != time[i])
-            left = 0.5 * ((amp[i] - amp[i - 1]) / (time[i] - time[i - 1]) + mid);
-        else left = 0.0;
-        left *= h;
-        if (i + 2 == size) right = mid;
-        else if (time[i + 1] != time[i + 2])
-            right = 0.5 * ((amp[i + 2] - amp[i + 1]) / (time[i + 2] - time[i + 1]) + mid);
-        else right = 0.0;
-        right *= h;
+        ss_interp_tangents(time, amp, size, i, &left, &right);
         const double p2 = p * p, p3 = p2 * p;
         out = amp[i] * (2 * p3 - 3 * p2 + 1) + amp[i + 1] * (-2 * p3 + 3 * p2)
             + left * (p3 - 2 * p2 + p) + right * (p3 - p2);
@@ -167,4 +387,63 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
         }
     }
     return out;
+}
+
+// One amplitude row of table tb at interval i, parameter p (i < 0: outside
+// the table), :2065-2142.
+__device__ static inline double ss_interp_row(const ss_table* tb, const double* amp, long long i, double p)
+{
+    const long long size = tb->size;
+    if (i < 0) return 0.0;
+    if (tb->order == 0) return amp[i];
+    if (i + 1 >= size) return 0.0;
+    if (tb->order == 1) return amp[i] * (1 - p) + amp[i + 1] * p;
+    double left, right;
+    ss_interp_tangents(tb->time, amp, size, i, &left, &right);
+    const double p2 = p * p, p3 = p2 * p;
+    return amp[i] * (2 * p3 - 3 * p2 + 1) + amp[i + 1] * (-2 * p3 + 3 * p2) + left * (p3 - 2 * p2 + p)
+           + right * (p3 - p2);
+}
+
+// Components [a0, a1) of multi-axis table k at time t into out[0 .. a1-a0),
+// from one index search; component a equals the one-axis sampler of amplitude
+// row a.  (The reference rejects 2-D amplitudes and suggests one sampler per
+// axis, spinsim/__init__.py:2062-2063; here the axes share the search.)
+__device__ static inline void ss_interp_sample_range(const ss_ctx* ctx, int k, double t, int a0, int a1,
+                                                     double* out)
+{
+    const ss_table* tb = &ctx->tables[k];
+    if (a1 > tb->axes) a1 = tb->axes;
+    const double* time = tb->time;
+    const double* amp0 = tb->amplitude + (long long)ctx->sim * tb->amplitude_stride;
+    const long long size = tb->size;
+    const long long last = size - 1;
+    if (size < 2) {
+        for (int a = a0; a < a1; ++a)
+            out[a - a0] = (size == 1 && !tb->zero_boundary) ? amp0[a * tb->axis_stride] : 0.0;
+        return;
+    }
+    if (!tb->zero_boundary && (t <= time[0] || t >= time[last])) {
+        const long long j = t <= time[0] ? 0 : last;
+        for (int a = a0; a < a1; ++a) out[a - a0] = amp0[a * tb->axis_stride + j];
+        return;
+    }
+    double p;
+    const long long i = ss_find_time_index(tb, t, &p);
+    for (int a = a0; a < a1; ++a) out[a - a0] = ss_interp_row(tb, amp0 + a * tb->axis_stride, i, p);
+}
+
+// The first n components of multi-axis table k at time t (n at most the
+// table's axes; further entries of out are left alone).
+__device__ static inline void ss_interp_sample_axes(const ss_ctx* ctx, int k, double t, double* out, int n)
+{
+    ss_interp_sample_range(ctx, k, t, 0, n, out);
+}
+
+// Component a of multi-axis table k at time t (0 beyond the table's axes).
+__device__ static inline double ss_interp_sample_axis(const ss_ctx* ctx, int k, int a, double t)
+{
+    double v = 0.0;
+    if (a >= 0) ss_interp_sample_range(ctx, k, t, a, a + 1, &v);
+    return v;
 }

@@ -11,7 +11,7 @@
 #include "device/ss_bloch.h"
 #include "device/ss_prelude.h"
 
-#define SS_HOST_ABI 1
+#define SS_HOST_ABI 3
 
 namespace {
 
@@ -87,16 +87,33 @@ int ss_host_spin(const double* state, double* spin, long long n, int dim, int th
 }
 
 // Sampler k of `tables` (row `sim` of a batched table) at n times: the
-// sampler of generate_interpolation_sampler, :2065-2142.
-int ss_host_interp_eval(const void* tables, int k, int sim, const double* t, double* out, long long n)
+// sampler of generate_interpolation_sampler, :2065-2142; `axes` components per
+// time, out (n, axes).
+int ss_host_interp_eval(const void* tables, int k, int sim, int axes, const double* t, double* out,
+                        long long n)
 {
-    if (n < 0 || k < 0 || sim < 0) return 1;
+    if (n < 0 || k < 0 || sim < 0 || axes < 1) return 1;
     if (n == 0) return 0;
     if (!tables || !t || !out) return 1;
     ss_ctx ctx;
     ctx.tables = static_cast<const ss_table*>(tables);
     ctx.sim = sim;
-    for (long long i = 0; i < n; ++i) out[i] = ss_interp_sample(&ctx, k, t[i]);
+    for (long long i = 0; i < n; ++i) {
+        if (axes == 1) out[i] = ss_interp_sample(&ctx, k, t[i]);
+        else ss_interp_sample_axes(&ctx, k, t[i], out + i * axes, axes);
+    }
+    return 0;
+}
+
+// The field callbacks' float32 transcendentals (precision fp32) at n points:
+// fn 0 sin, 1 cos, 2 exp (ss_sin32 / ss_cos32 / ss_exp32 of ss_prelude.h).
+int ss_host_f32_eval(int fn, const double* x, double* out, long long n)
+{
+    if (n < 0 || fn < 0 || fn > 2) return 1;
+    if (n == 0) return 0;
+    if (!x || !out) return 1;
+    for (long long i = 0; i < n; ++i)
+        out[i] = fn == 0 ? ss_sin32(x[i]) : fn == 1 ? ss_cos32(x[i]) : ss_exp32(x[i]);
     return 0;
 }
 

@@ -13,6 +13,7 @@
 #define __device__
 #define __host__
 #define __forceinline__ inline __attribute__((always_inline))
+#define __constant__
 
 struct double2 {
     double x, y;

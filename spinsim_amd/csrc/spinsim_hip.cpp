@@ -49,8 +49,9 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 7;
-static_assert(sizeof(::ss_table) == 40, "ss_table layout");
+constexpr int kAbiVersion = 8;
+static_assert(sizeof(::ss_table) == 56, "ss_table layout");
+static_assert(sizeof(::ss_table) == sizeof(dev::ss_table), "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
 
 // hipRTC options of every integrator compile (packed FP32 instructions off).
@@ -926,16 +927,54 @@ __global__ void __launch_bounds__(256) zero_doubles(double* p, long long n, int*
 }
 
 
-// Evaluate sampler k of `tables` at n times (row `sim` of a batched table).
-__global__ void __launch_bounds__(256) interp_kernel(const dev::ss_table* tables, int k, int sim, const double* t,
-                                                     double* out, long long n)
+// Deterministic adjoint gradients: entry j of simulation s's gradient row is
+// the sum, over the controls whose entries cover j and over the simulation's
+// rows in order, of the rows' window records (ss_batch_adjoint_kernel in its
+// deterministic mode).  Window starts never decrease along a simulation's
+// rows, so the rows whose window holds knot k are found by bisection.
+__global__ void __launch_bounds__(256) adjoint_reduce_windows(const double* partial, const long long* window_base,
+                                                              const long long* offsets, const dev::ss_table* tables,
+                                                              ss_control controls_0, ss_control controls_1,
+                                                              ss_control controls_2, ss_control controls_3,
+                                                              int n_controls, long long n_sims, long long grad_stride,
+                                                              double* grad)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sims * grad_stride) return;
+    const long long sim = i / grad_stride, j = i - sim * grad_stride;
+    const ss_control ctl[4] = {controls_0, controls_1, controls_2, controls_3};
+    const long long r0 = offsets[sim], r1 = offsets[sim + 1];
+    double sum = 0.0;
+    for (int c = 0; c < n_controls; ++c) {
+        const long long k = j - ctl[c].grad_offset;
+        if (k < 0 || k >= tables[ctl[c].table].size) continue;
+        // first row whose window reaches knot k: base + W > k
+        long long lo = r0, hi = r1;
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            if (window_base[mid * n_controls + c] + SS_ADJ_WINDOW > k) hi = mid; else lo = mid + 1;
+        }
+        for (long long r = lo; r < r1; ++r) {
+            const long long b = window_base[r * n_controls + c];
+            if (b > k) break;
+            sum += partial[(r * n_controls + c) * SS_ADJ_WINDOW + (k - b)];
+        }
+    }
+    grad[i] = sum;
+}
+
+// Evaluate sampler k of `tables` at n times (row `sim` of a batched table):
+// `axes` components per time, out (n, axes).
+__global__ void __launch_bounds__(256) interp_kernel(const dev::ss_table* tables, int k, int sim, int axes,
+                                                     const double* t, double* out, long long n)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     dev::ss_ctx ctx;
     ctx.tables = tables;
     ctx.sim = sim;
-    out[i] = dev::ss_interp_sample(&ctx, k, t[i]);
+    if (axes == 1) out[i] = dev::ss_interp_sample(&ctx, k, t[i]);
+    else dev::ss_interp_sample_axes(&ctx, k, t[i], out + i * axes, axes);
 }
 
 __global__ void zero_counter(unsigned long long* p)
@@ -1030,8 +1069,12 @@ std::string build_source(const char* field_src, int dim, int method, int expo, i
     s += ss_src_args;
     s += ss_src_complex;
     s += ss_src_prelude;
+    // precision fp32: the callback's sin, cos and exp in float32 (ss_prelude.h)
+    const bool f32 = precision == SS_PRECISION_FP32;
+    if (f32) s += "\n#define sin(x) ss_sin32(x)\n#define cos(x) ss_cos32(x)\n#define exp(x) ss_exp32(x)\n";
     s += "\n#line 1 \"get_field\"\n";
     s += field_src;
+    if (f32) s += "\n#undef sin\n#undef cos\n#undef exp\n";
     s += "\n#line 1 \"ss_integrator.h\"\n";
     s += ss_src_integrator;
     return s;
@@ -1086,7 +1129,7 @@ int ss_compile_code(const char* field_src, int dim, int method, int expo, int ro
                     "eight-generator exponentiation");
     if (number_of_squares > 60) return fail("ss_compile_code: number_of_squares %d too large", number_of_squares);
     if (precision != SS_PRECISION_FP64 && precision != SS_PRECISION_FP32_BULK &&
-        precision != SS_PRECISION_FP64_SERIES)
+        precision != SS_PRECISION_FP64_SERIES && precision != SS_PRECISION_FP32)
         return fail("ss_compile_code: bad precision %d", precision);
     const std::string src = build_source(field_src, dim, method, expo, rotating, number_of_squares,
                                          field_has_context, precision);
@@ -1530,7 +1573,7 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
                      const int32_t* n_fine, int64_t total, const double* u, const double* state,
                      const double* costate, const ss_table* tables, int n_tables,
                      const ss_control_desc* controls, int n_controls, double* grad, int64_t grad_stride,
-                     int32_t* status, ss_stream stream)
+                     int32_t* status, void* workspace, size_t workspace_bytes, ss_stream stream)
 {
     if (!m) return fail("ss_batch_adjoint: null module");
     if (m->expo == SS_EXPO_LIE_TROTTER_8)
@@ -1586,6 +1629,14 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
         a.controls[c].scale = controls[c].scale;
         a.controls[c].grad_offset = controls[c].grad_offset;
     }
+    if (workspace) {
+        const size_t need = ss_batch_adjoint_workspace(total, n_controls);
+        if (workspace_bytes < need)
+            return fail("ss_batch_adjoint: deterministic workspace of %zu bytes, need %zu", workspace_bytes, need);
+        a.partial = static_cast<double*>(workspace);
+        a.window_base = reinterpret_cast<long long*>(static_cast<char*>(workspace) +
+                                                     align256(size_t(total) * n_controls * SS_ADJ_WINDOW * 8));
+    }
     const long long blocks = (total + SS_ADJ_BLOCK - 1) / SS_ADJ_BLOCK;
     if (blocks > 2147483647LL) return fail("ss_batch_adjoint: grid too large");
     // the per-lane gradient windows: only as much LDS as the controls need,
@@ -1594,20 +1645,37 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
     void* params[] = {&a};
     SS_HIP(hipModuleLaunchKernel(m->adjoint, unsigned(blocks), 1, 1, SS_ADJ_BLOCK, 1, 1, lds, s, params,
                                  nullptr));
+    if (workspace) {
+        const long long n = n_sims * grad_stride;
+        const long long rb = (n + 255) / 256;
+        if (rb > 2147483647LL) return fail("ss_batch_adjoint: gradient too large");
+        hipLaunchKernelGGL(adjoint_reduce_windows, dim3(unsigned(rb)), dim3(256), 0, s, a.partial, a.window_base,
+                           a.offsets, reinterpret_cast<const dev::ss_table*>(tables), a.controls[0], a.controls[1],
+                           a.controls[2], a.controls[3], n_controls, (long long)n_sims, (long long)grad_stride,
+                           grad);
+        SS_HIP(hipGetLastError());
+    }
     return 0;
 }
 
-int ss_interp_eval(const ss_table* tables, int k, int sim, const double* t, double* out, int64_t n,
+size_t ss_batch_adjoint_workspace(int64_t total, int n_controls)
+{
+    if (total < 0 || n_controls < 1 || n_controls > SS_MAX_CONTROLS) return 0;
+    return align256(size_t(total) * n_controls * SS_ADJ_WINDOW * 8) + align256(size_t(total) * n_controls * 8);
+}
+
+int ss_interp_eval(const ss_table* tables, int k, int sim, int axes, const double* t, double* out, int64_t n,
                    ss_stream stream)
 {
     if (n < 0) return fail("ss_interp_eval: negative n");
     if (n == 0) return 0;
     if (!tables || !t || !out) return fail("ss_interp_eval: null buffer");
     if (k < 0 || sim < 0) return fail("ss_interp_eval: negative table or row index");
+    if (axes < 1) return fail("ss_interp_eval: axes must be at least 1");
     const long long blocks = (n + 255) / 256;
     if (blocks > 2147483647LL) return fail("ss_interp_eval: too many samples");
     hipLaunchKernelGGL(interp_kernel, dim3(unsigned(blocks)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       reinterpret_cast<const dev::ss_table*>(tables), k, sim, t, out, (long long)n);
+                       reinterpret_cast<const dev::ss_table*>(tables), k, sim, axes, t, out, (long long)n);
     SS_HIP(hipGetLastError());
     return 0;
 }

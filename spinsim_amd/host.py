@@ -39,7 +39,7 @@ MODULE_FLAGS = ["-O3", "-std=c++17", "-march=native", "-ffp-contract=off", "-fop
                 "-fPIC", "-Wno-unknown-pragmas", "-Wno-unused-function"]
 HOST_LIB_FLAGS = ["-O3", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-shared", "-fPIC",
                   "-Wno-unknown-pragmas", "-Wno-unused-function"]
-HOST_ABI = 1
+HOST_ABI = 3
 MODULE_ABI = 1
 
 _lock = threading.Lock()
@@ -90,8 +90,10 @@ def host_lib():
             lib.ss_host_state.restype = c_int
             lib.ss_host_spin.argtypes = [c_vp, c_vp, c_ll, c_int, c_int]
             lib.ss_host_spin.restype = c_int
-            lib.ss_host_interp_eval.argtypes = [c_vp, c_int, c_int, c_vp, c_vp, c_ll]
+            lib.ss_host_interp_eval.argtypes = [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll]
             lib.ss_host_interp_eval.restype = c_int
+            lib.ss_host_f32_eval.argtypes = [c_int, c_vp, c_vp, c_ll]
+            lib.ss_host_f32_eval.restype = c_int
             _host_lib = lib
     return _host_lib
 
@@ -128,9 +130,14 @@ def module_source(field_src, dim, method, expo, rotating, number_of_squares, has
             f"#define SS_EXPO {expo}\n#define SS_ROT {1 if rotating else 0}\n"
             f"#define SS_HYPER {hyper}\n#define SS_CTX {1 if has_context else 0}\n"
             f"#define SS_PREC {precision}\n")
+    # precision fp32: the callback's sin, cos and exp in float32 (ss_prelude.h)
+    f32_on = "#define sin(x) ss_sin32(x)\n#define cos(x) ss_cos32(x)\n#define exp(x) ss_exp32(x)\n"
+    f32_off = "#undef sin\n#undef cos\n#undef exp\n"
+    f32 = precision == 3
     return (head + '#include "host/ss_host_prelude.h"\n#include "device/ss_args.h"\n'
             '#include "device/ss_complex.h"\n#include "device/ss_prelude.h"\n'
-            '#line 1 "get_field"\n' + field_src + "\n"
+            + (f32_on if f32 else "") + '#line 1 "get_field"\n' + field_src + "\n"
+            + (f32_off if f32 else "") +
             '#include "device/ss_integrator.h"\n#include "host/ss_host_module.h"\n')
 
 
@@ -211,6 +218,16 @@ def compile_module(field_src, dim, method, expo, rotating, number_of_squares, ha
     with _lock:
         _modules[key] = mod
     return mod
+
+
+def f32_eval(fn, x):
+    """The field callbacks' float32 sin / cos / exp (precision "fp32"), host
+    build: fn in ("sin", "cos", "exp")."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, np.float64)))
+    out = np.empty_like(x)
+    _check(host_lib().ss_host_f32_eval(("sin", "cos", "exp").index(fn), x.ctypes.data,
+                                       out.ctypes.data, x.size), "ss_host_f32_eval")
+    return out
 
 
 def state(psi0, u, with_spin=False):

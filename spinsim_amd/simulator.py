@@ -38,7 +38,7 @@ def _cache_dir():
     return d
 
 
-PRECISIONS = {"fp64": 0, "fp32_bulk": 1, "fp64_series": 2}
+PRECISIONS = {"fp64": 0, "fp32_bulk": 1, "fp64_series": 2, "fp32": 3}
 
 
 def compile_field(src: str, dim: int, method: int, expo: int, rotating: bool,
@@ -81,7 +81,9 @@ class FieldSource:
     ``__device__ void get_field(double t, const double* sweep, double* field)``
     (``..., SS_CONTEXT)`` when it reads samplers), or a FieldSource.
     ``samplers`` maps names usable in HIP-C text to samplers (or is a list,
-    read with ``SS_SAMPLE(k, t)``).
+    read with ``SS_SAMPLE(k, t)``); a multi-axis sampler of A components reads
+    as ``name(t, out)`` into ``double out[A]`` (or ``SS_SAMPLE_AXES(k, t, out,
+    A)`` / ``SS_SAMPLE_AXIS(k, a, t)``).
     """
 
     def __init__(self, get_field, samplers=None):
@@ -99,7 +101,12 @@ class FieldSource:
             for s in self.samplers:
                 if not isinstance(s, InterpolationSampler):
                     raise TypeError("samplers must be InterpolationSampler objects")
-            macros = "".join(f"#define {n}(t) SS_SAMPLE({k}, (t))\n" for k, n in enumerate(names))
+            # a one-axis sampler reads as name(t); a multi-axis one as name(t, out),
+            # filling out[0 .. A) from one index search
+            macros = "".join(
+                f"#define {n}(t) SS_SAMPLE({k}, (t))\n" if self.samplers[k].axes == 1 else
+                f"#define {n}(t, out) SS_SAMPLE_AXES({k}, (t), (out), {self.samplers[k].axes})\n"
+                for k, n in enumerate(names))
             self.source = macros + get_field
             self.has_context = bool(self.samplers) or "SS_CONTEXT" in get_field
             return
@@ -202,14 +209,18 @@ class Simulator:
     waves in phases of decreasing item length so they finish together; -k
     for the same on k waves per SIMD; ``"auto"`` to time the candidates on
     first use of each shape) and
-    ``precision``: ``"fp64"`` (default; the reference's arithmetic) or
+    ``precision``: ``"fp64"`` (default; the reference's arithmetic),
     ``"fp32_bulk"`` -- the higher-order residual exp(A) - I - A of each step
     operator and the products of residuals within a lane's chunk of steps in
     float32 (a scaled Taylor series instead of the exponentiation method),
     while field samples, the frame, first-order terms, the products across
-    lanes, the outputs and the state scan stay float64.  Accurate when each
-    step's rotation angle is small (rotating frame); check with
-    :func:`spinsim_amd.precision_error`.
+    lanes, the outputs and the state scan stay float64 -- or ``"fp32"``:
+    float32-bulk, and the field function's sin, cos and exp evaluated in
+    float32 on arguments reduced in float64 (its other arithmetic, the time
+    and the frame stay float64).  Accurate when each step's rotation angle
+    is small (rotating frame); check with :func:`spinsim_amd.precision_error`.
+    ``"fp64_series"`` runs the float32-bulk series in float64 (for
+    measurement).
     ``threads_per_block`` and ``max_registers`` are accepted for
     compatibility and do not change the gfx950 launch.
 
@@ -259,7 +270,7 @@ class Simulator:
             self.step_exponential = self.exponentiation_method.value
         else:
             self.step_exponential = ("residual series, " +
-                                     ("float32" if precision == "fp32_bulk" else "float64"))
+                                     ("float64" if precision == "fp64_series" else "float32"))
             if exponentiation_method is not None or number_of_squares != 24:
                 warnings.warn(
                     f"precision={precision!r} computes each step operator as exp(A) = I + A + "

@@ -16,7 +16,8 @@ real functions of ``math`` and ``numpy`` (sin, cos, exp, sqrt, atan2, ...),
 ``abs/min/max/float/int``; numeric globals and closure constants (inlined as
 literals, frozen at compile time like numba freezes them); interpolation
 samplers from ``generate_interpolation_sampler`` (read at run time from
-device tables); and other module-level Python functions that return a value
+device tables; a multi-axis sampler's components as ``v = sampler(t)`` then
+``v[a]``, or ``sampler(t)[a]``); and other module-level Python functions that return a value
 and obey the same rules (emitted as ``__device__`` helpers).  Anything else
 raises ``TranspileError`` naming the line.
 """
@@ -94,13 +95,37 @@ class _Fn:
                     pass
         self.locals: dict[str, str] = {}  # name -> C type
         self.int_vars: set[str] = set()
+        self.arrays: dict[str, int] = {}  # name -> components (multi-axis sampler results)
 
     def err(self, node, msg):
         line = self.first_line + getattr(node, "lineno", 1) - 1
         return TranspileError(f"{self.func.__name__} (line {line}): {msg}")
 
     # ---- declarations -----------------------------------------------------
+    def _sampler_of(self, node):
+        """The sampler a Call node calls, if any."""
+        from .interpolation import InterpolationSampler
+
+        if not isinstance(node, ast.Call):
+            return None
+        f = node.func
+        obj = None
+        if isinstance(f, ast.Name):
+            obj = self.globals.get(f.id)
+        elif isinstance(f, ast.Attribute) and self._module_of(f.value) is None:
+            obj = self._resolve(f)
+        return obj if isinstance(obj, InterpolationSampler) else None
+
     def _collect(self):
+        for node in ast.walk(self.node):
+            if isinstance(node, ast.Assign) and len(node.targets) == 1 \
+                    and isinstance(node.targets[0], ast.Name):
+                sampler = self._sampler_of(node.value)
+                if sampler is not None and sampler.axes > 1:
+                    name = node.targets[0].id
+                    if self.arrays.get(name, sampler.axes) != sampler.axes:
+                        raise self.err(node, f"{name!r} holds samples of different axis counts")
+                    self.arrays[name] = sampler.axes
         for node in ast.walk(self.node):
             if isinstance(node, ast.For):
                 if isinstance(node.target, ast.Name):
@@ -118,11 +143,14 @@ class _Fn:
                         self.locals.setdefault(n.id, "double")
         for v in self.int_vars:
             self.locals[v] = "long long"
+        for n in self.arrays:
+            self.locals.pop(n, None)
 
     def emit(self) -> str:
         self._collect()
         body = self._block(self.node.body, 1)
         decls = "".join(f"    {t} {n} = 0;\n" for n, t in sorted(self.locals.items()))
+        decls += "".join(f"    double {n}[{a}] = {{0}};\n" for n, a in sorted(self.arrays.items()))
         if self.role == "field":
             if len(self.params) != 3:
                 raise self.err(self.node, "get_field must take (time_sample, sweep_parameters, "
@@ -159,6 +187,16 @@ class _Fn:
                 raise self.err(st, "get_field returns nothing; write into field_sample")
             return pad + f"return {self._expr(st.value)};\n"
         if isinstance(st, ast.Assign):
+            if len(st.targets) == 1 and isinstance(st.targets[0], ast.Name) \
+                    and st.targets[0].id in self.arrays:
+                name = st.targets[0].id
+                sampler = self._sampler_of(st.value)
+                if sampler is None or sampler.axes != self.arrays[name] or len(st.value.args) != 1:
+                    raise self.err(st, f"{name!r} holds multi-axis samples; assign it only "
+                                       "from one call of that sampler")
+                slot = self.ctx.sampler_slot(sampler)
+                t = self._expr(st.value.args[0])
+                return pad + f"SS_SAMPLE_AXES({slot}, {t}, {name}, {sampler.axes});\n"
             value = self._expr(st.value)
             if len(st.targets) == 1 and isinstance(st.targets[0], ast.Tuple):
                 raise self.err(st, "tuple assignment is not supported")
@@ -207,6 +245,8 @@ class _Fn:
 
     def _target(self, t) -> str:
         if isinstance(t, ast.Name):
+            if t.id in self.arrays:
+                raise self.err(t, f"{t.id!r} holds multi-axis samples and cannot be updated")
             return t.id
         if isinstance(t, ast.Subscript) and isinstance(t.value, ast.Name):
             if self.role == "field" and t.value.id == self.params[2]:
@@ -292,13 +332,41 @@ class _Fn:
             if isinstance(node.value, ast.Name) and self.role == "field" and \
                     node.value.id in self.params[1:]:
                 return f"{node.value.id}[{self._int_expr(self._index(node))}]"
-            raise self.err(node, "only sweep_parameters[k] and field_sample[k] can be indexed")
+            if isinstance(node.value, ast.Name) and node.value.id in self.arrays:
+                return self._axis_read(node, node.value.id, self.arrays[node.value.id])
+            sampler = self._sampler_of(node.value)
+            if sampler is not None and sampler.axes > 1:
+                if len(node.value.args) != 1:
+                    raise self.err(node, "a sampler takes one argument (the time)")
+                slot = self.ctx.sampler_slot(sampler)
+                axis = self._index(node)
+                if isinstance(axis, ast.Constant):
+                    self._axis_constant(node, axis, sampler.axes)
+                return (f"SS_SAMPLE_AXIS({slot}, (int)({self._int_expr(axis)}), "
+                        f"{self._expr(node.value.args[0])})")
+            raise self.err(node, "only sweep_parameters[k], field_sample[k] and multi-axis "
+                                 "samples can be indexed")
         if isinstance(node, ast.Call):
             return self._call(node)
         raise self.err(node, f"expression {type(node).__name__} is not supported")
 
+    def _axis_constant(self, node, axis, axes):
+        if not (isinstance(axis.value, int) and not isinstance(axis.value, bool)
+                and 0 <= axis.value < axes):
+            raise self.err(node, f"axis index {axis.value!r} out of range for {axes} axes")
+
+    def _axis_read(self, node, name, axes) -> str:
+        axis = self._index(node)
+        if isinstance(axis, ast.Constant):
+            self._axis_constant(node, axis, axes)
+            return f"{name}[{axis.value}]"
+        i = self._int_expr(axis)
+        return f"(({i}) >= 0 && ({i}) < {axes} ? {name}[{i}] : 0.0)"
+
     def _name(self, node) -> str:
         n = node.id
+        if n in self.arrays:
+            raise self.err(node, f"{n!r} holds multi-axis samples; read them as {n}[a]")
         if n in self.params:
             if self.role == "field" and n != self.params[0]:
                 raise self.err(node, f"{n!r} can only be used with an index")
@@ -407,6 +475,9 @@ class _Fn:
         if isinstance(obj, InterpolationSampler):
             if len(args) != 1:
                 raise self.err(node, "a sampler takes one argument (the time)")
+            if obj.axes > 1:
+                raise self.err(node, f"a multi-axis sampler gives {obj.axes} values: assign them "
+                                     "to a name (v = sampler(t), then v[a]) or index the call")
             slot = self.ctx.sampler_slot(obj)
             return f"SS_SAMPLE({slot}, {args[0]})"
         if inspect.isfunction(obj):

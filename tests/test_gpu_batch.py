@@ -59,7 +59,7 @@ def test_uneven_batch_matches_oracle(field, lanes):
 
 
 def test_batch_sampler_rows():
-    """A (B, M) amplitude table: simulation s reads row s."""
+    """A per-simulation (B, M) amplitude table: simulation s reads row s."""
     import spinsim_amd as ss
 
     print("case: batch sampler rows", flush=True)
@@ -67,7 +67,7 @@ def test_batch_sampler_rows():
     b, m = 6, 120
     tk = np.linspace(0, 2e-4, m)
     cx = rng.normal(0, 2 * math.pi * 5e3, (b, m))
-    ctl = ss.generate_interpolation_sampler(tk, cx)
+    ctl = ss.generate_interpolation_sampler(tk, cx, per_simulation=True)
 
     def get_field(time_sample, sweep_parameters, field_sample):
         field_sample[0] = ctl(time_sample)

@@ -2,10 +2,10 @@
 
 * The 4096-simulation spin-half Rabi ensemble (durations 1-50 ms, 50 ns
   steps, 1 us outputs: 1.06e8 coarse rows, a 6.8 GB operator array, so rows
-  late in the CSR range sit beyond 2^32 bytes), float64 and float32-bulk,
+  late in the CSR range sit beyond 2^32 bytes), float64, float32-bulk and fp32,
   in one persistent launch and one batch scan.  64 simulations -- the first,
   the last, the longest, the shortest and the rest spread over the range --
-  are checked against the C oracle (float64 1e-9, float32-bulk 1e-6), and
+  are checked against the C oracle (float64 1e-9, float32 modes 1e-6), and
   the last simulation's operators and Bloch vectors too.
 * The differentiation workload: 1024 spin-one pulses of 2 ms at 20 ns steps,
   x and y from 2000 cubic control knots each (the 2000-knot gradient
@@ -50,7 +50,7 @@ def _checked(dur, count=64):
     return sorted(picks)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32_bulk", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32_bulk", 1e-6), ("fp32", 1e-6)])
 def test_ensemble_4096_full_size(precision, tol):
     import torch
 

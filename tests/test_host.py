@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == _native.ABI_VERSION == 7
+    assert lib.ss_abi_version() == _native.ABI_VERSION == 8
     assert lib.ss_source_hash() != 0
 
 
@@ -273,8 +273,14 @@ def test_sampler_validation():
         ss.generate_interpolation_sampler([0, 1], [0, 1], order=2)
     with pytest.raises(ValueError):
         ss.generate_interpolation_sampler([0, 1, 2], [0, 1])
-    s = ss.generate_interpolation_sampler([0, 1, 2], np.ones((4, 3)))
-    assert s.rows == 4
+    s = ss.generate_interpolation_sampler([0, 1, 2], np.ones((4, 3)), per_simulation=True)
+    assert s.rows == 4 and s.axes == 1
+    m = ss.generate_interpolation_sampler([0, 1, 2], np.ones((4, 3)))
+    assert m.rows == 1 and m.axes == 4
+    with pytest.raises(ValueError):
+        ss.generate_interpolation_sampler([0, 1, 2], np.ones((2, 4, 3)))
+    b = ss.generate_interpolation_sampler([0, 1, 2], np.ones((2, 4, 3)), per_simulation=True)
+    assert b.rows == 2 and b.axes == 4
     s.update(np.zeros((4, 3)))
     assert s.version == 1
     with pytest.raises(ValueError):
