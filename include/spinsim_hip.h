@@ -192,6 +192,19 @@ int ss_batch_state_scan(const double *psi0, const double *u, double *state, doub
                         int64_t chunk_rows, void *workspace, size_t workspace_bytes,
                         ss_stream stream);
 
+/* ss_batch_state_scan without a workspace argument and without allocating:
+ * the chunk records live in a static device array of the library (416 KiB
+ * per GPU, held until the process ends), so the scan takes at most 2048
+ * chunks -- always true of the default chunking (chunk_rows 0); a chunk_rows
+ * giving more is rejected.  Results equal ss_batch_state_scan's with the same
+ * chunk_rows.  Calls that overlap in time on one GPU (different streams, or
+ * host threads) share that array and race; calls in stream order do not.
+ * (No reference counterpart: for callers that cannot allocate device memory,
+ * e.g. code under HIP graph capture or a serving loop that forbids it.) */
+int ss_batch_state_scan_static(const double *psi0, const double *u, double *state, double *spin,
+                               const int64_t *offsets, int64_t n_sims, int64_t total, int dim,
+                               int64_t chunk_rows, ss_stream stream);
+
 /* Costates for the adjoint: costate[o+n-1] = lambda_final[s] and
  * costate[i] = u[i+1]^+ costate[i+1] over each simulation's rows, i.e. the
  * costate at the END of every coarse interval.  lambda_final: device (n_sims,

@@ -48,6 +48,8 @@ SIGNATURES = {
     "ss_time_evolution_plan": (c_int, [c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
                                     c_vp, ctypes.c_size_t, c_vp]),
+    "ss_batch_state_scan_static": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int,
+                                           c_i64, c_vp]),
     "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
                                       c_vp, ctypes.c_size_t, c_vp]),
     "ss_batch_final_state": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp]),

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1456,6 +1457,36 @@ int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, int64_t chun
     return 0;
 }
 
+// Scratch of ss_batch_state_scan_static: the batch scan's chunk records (map,
+// constant term and a flag per workgroup chunk), for callers that cannot
+// allocate device memory.  Sized for kStaticScanChunks spin-one records
+// (416 KiB); every GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes the array's name to a kernel: it
+// passes the device address hipGetSymbolAddress gives for the current device.
+constexpr long long kStaticScanChunks = 2048;
+static __device__ __align__(256) double scratch[kStaticScanChunks * scan_agg_stride(3)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int static_scratch(void** out)
+{
+    int dev = 0;
+    SS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail("device ordinal %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail("hipGetSymbolAddress gave a null address for the scan scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace
 
 int ss_state_scan(const double* psi0, const double* u, double* state, double* spin, int64_t n, int dim,
@@ -1525,6 +1556,39 @@ int ss_batch_state_scan(const double* psi0, const double* u, double* state, doub
     a.n_sims = n_sims;
     return launch_scan("ss_batch_state_scan", a, dim, false, chunk_rows, workspace, workspace_bytes,
                         static_cast<hipStream_t>(stream));
+}
+
+int ss_batch_state_scan_static(const double* psi0, const double* u, double* state, double* spin,
+                               const int64_t* offsets, int64_t n_sims, int64_t total, int dim,
+                               int64_t chunk_rows, ss_stream stream)
+{
+    const char* who = "ss_batch_state_scan_static";
+    if (dim != 2 && dim != 3) return fail("%s: dim must be 2 or 3", who);
+    if (n_sims < 0 || total < 0) return fail("%s: negative size", who);
+    if (n_sims == 0 || total == 0) return 0;
+    if (!psi0 || !u || !state || !offsets) return fail("%s: null buffer", who);
+    if (n_sims > 2147483647LL) return fail("%s: too many simulations", who);
+    if (chunk_rows < 0 || chunk_rows % kScanTileRows != 0)
+        return fail("%s: chunk_rows must be 0 (default) or a positive multiple of %lld, got %lld", who,
+                    kScanTileRows, (long long)chunk_rows);
+    const ScanLayout l = scan_layout(total, dim, chunk_rows);
+    if (l.chunks > kStaticScanChunks)
+        return fail("%s: %lld rows in chunks of %lld rows make %lld chunks; the static scratch holds "
+                    "at most %lld (use a larger chunk_rows, the default chunking, or "
+                    "ss_batch_state_scan with a workspace)",
+                    who, (long long)total, l.chunk_rows, l.chunks, kStaticScanChunks);
+    void* ws = nullptr;
+    if (int rc = static_scratch(&ws)) return rc;
+    ScanArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = u;
+    a.out = state;
+    a.spin = spin;
+    a.offsets = reinterpret_cast<const long long*>(offsets);
+    a.init = psi0;
+    a.total = total;
+    a.n_sims = n_sims;
+    return launch_scan(who, a, dim, false, chunk_rows, ws, sizeof(scratch), static_cast<hipStream_t>(stream));
 }
 
 int ss_batch_costate_scan(const double* lambda_final, const double* u, double* costate, const int64_t* offsets,
