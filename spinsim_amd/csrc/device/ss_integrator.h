@@ -1090,8 +1090,8 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
 // f the transformed sample, psi and lam at the interval's end; the sum is
 // scattered onto the z controls at the midpoint.
 // ===========================================================================
-#define SS_ADJ_ORDER 6       // Frechet series order (terms up to n = 6)
-#define SS_ADJ_TAYLOR 9      // exp(-A) v Taylor order
+#define SS_ADJ_ORDER 5       // Frechet series order (terms up to n = 5)
+#define SS_ADJ_TAYLOR 8      // exp(-A) v Taylor order
 #define SS_ADJ_NU 0.05       // sub-step until the generator norm bound is below this
 #define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 51)
 
@@ -1134,7 +1134,8 @@ static __device__ __forceinline__ void ss_gen_vec(const ss_gen64& a, double s, c
     }
 }
 
-// v <- exp(-A) v by its Taylor series (|A| <= SS_ADJ_NU: truncation < 1e-17).
+// v <- exp(-A) v by its Taylor series (|A| <= SS_ADJ_NU: truncation below
+// nu^9 / 9! = 5.4e-18 relative).
 static __device__ __forceinline__ void ss_expm_neg_vec(const ss_gen64& a, ss_cd v[SS_D])
 {
     ss_cd term[SS_D], acc[SS_D];
@@ -1169,7 +1170,10 @@ static __device__ __forceinline__ double ss_re_dot(const ss_cd a[SS_D], const ss
 //     Re(lam^+ L(A, E) psi) = sum_{p+q<=N} c_{p+q} Re(alpha_p^+ E beta_q),
 // alpha_p = (A^+)^p lam = (-A)^p lam, beta_q = A^q psi, c_n = 1/(n+1)!; both
 // power sequences once (2N generator products), then for each q the
-// combination gamma_q = sum_p c_{p+q} alpha_p (vector sums only).
+// combination gamma_q = sum_p c_{p+q} alpha_p (vector sums only).  The
+// series left out is below nu^(N+1) / (N+1)! of |E| relative: 2.2e-11 at the
+// sub-step bound nu = 0.05, 2e-21 at the 1e-3 of a 20 ns step in a 2 pi 5 kHz
+// field.
 static __device__ __forceinline__ void ss_frechet_grad(const ss_gen64& a, const ss_cd lam[SS_D],
                                                        const ss_cd psi[SS_D], unsigned mask, double w,
                                                        double grad[SS_LIE])
@@ -1185,7 +1189,7 @@ static __device__ __forceinline__ void ss_frechet_grad(const ss_gen64& a, const 
         ss_gen_vec(a, 1.0, beta[q - 1], beta[q]);
         ss_gen_vec(a, -1.0, alpha[q - 1], alpha[q]);
     }
-    const double c[SS_ADJ_ORDER + 1] = {1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    const double c[7] = {1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
 #pragma unroll
     for (int q = 0; q <= SS_ADJ_ORDER; ++q) {
         ss_cd gam[SS_D];
@@ -1434,16 +1438,34 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
 #endif
     const double half = 0.5 * dt;
     const double off_a = half * a.off_a_frac, off_b = half * a.off_b_frac;
+    // the last step's fine time by the forward pass's accumulation (:819, :836),
+    // then stepped back by subtraction (within an ulp of the forward's times);
+    // the windings follow by multiplication like the forward pass's
+    double tf = tc;
+#pragma unroll 1
+    for (int k = 0; k + 1 < n_fine; ++k) tf += dt;
+    const ss_cd back = ss_conj(ss_winding(rot, dt));
+    ss_wind wa, wb;
+    wa.t_rel = (tf + off_a) - tc;
+    wa.w = ss_winding(rot, wa.t_rel);
+    wb.t_rel = (tf + off_b) - tc;
+#if SS_METHOD != 1
+    wb.w = ss_winding(rot, wb.t_rel);
+#endif
 #pragma unroll 1
     for (int j = n_fine - 1; j >= 0; --j) {
-        double tf = tc;
-#pragma unroll 1
-        for (int k = 0; k < j; ++k) tf += dt;
+        if (j < n_fine - 1) {
+            tf -= dt;
+            ss_wind_advance(wa, (tf + off_a) - tc, back, rot, -dt);
+#if SS_METHOD != 1
+            ss_wind_advance(wb, (tf + off_b) - tc, back, rot, -dt);
+#endif
+        }
         double fa[SS_FIELD_LEN], g1[SS_FIELD_LEN], d1[SS_LIE], dfa[SS_LIE], lab[4];
-        const double tra = (tf + off_a) - tc;
-        const ss_cd wa = ss_winding(rot, tra);
+        const double tra = wa.t_rel;
+        const ss_cd wa_w = wa.w;
         ss_call_field(tra + tc, sweep, fa, &ctx);
-        ss_frame(fa, rot, wa);
+        ss_frame(fa, rot, wa_w);
 #pragma unroll
         for (int k = 0; k < SS_LIE; ++k) d1[k] = 0.0;
 #if SS_METHOD == 1
@@ -1455,14 +1477,14 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
 #if SS_ROT
         if (z_control) dz_mid += tra * (dfa[0] * fa[1] - dfa[1] * fa[0]) - dfa[2];
 #endif
-        ss_unframe(dfa, wa, lab);
+        ss_unframe(dfa, wa_w, lab);
         ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
 #else
         double fb[SS_FIELD_LEN], g2[SS_FIELD_LEN], d2[SS_LIE], dfb[SS_LIE];
-        const double trb = (tf + off_b) - tc;
-        const ss_cd wb = ss_winding(rot, trb);
+        const double trb = wb.t_rel;
+        const ss_cd wb_w = wb.w;
         ss_call_field(trb + tc, sweep, fb, &ctx);
-        ss_frame(fb, rot, wb);
+        ss_frame(fb, rot, wb_w);
 #pragma unroll
         for (int k = 0; k < SS_LIE; ++k) d2[k] = 0.0;
 #if SS_METHOD == 0
@@ -1497,9 +1519,9 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
             dz_mid += tra * (dfa[0] * fa[1] - dfa[1] * fa[0]) - dfa[2] +
                       trb * (dfb[0] * fb[1] - dfb[1] * fb[0]) - dfb[2];
 #endif
-        ss_unframe(dfa, wa, lab);
+        ss_unframe(dfa, wa_w, lab);
         ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
-        ss_unframe(dfb, wb, lab);
+        ss_unframe(dfb, wb_w, lab);
         ss_scatter_sample(a, &ctx, trb + tc, lab, win, base, grad_row);
 #endif
     }

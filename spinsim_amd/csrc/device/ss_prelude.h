@@ -76,15 +76,15 @@ struct ss_ctx {
 //     cm = cos b - 1 = -b^2/2 + b^4/24, sm = sin b / b - 1 = -b^2/6 + b^4/120
 //     in float32 (|cm| < 7.6e-5, |sm| < 2.6e-5: their rounding is about 1e-11
 //     absolute in the result; truncation below 1e-17).
-//   exp: x = n ln2 + r, |r| <= ln2 / 2, r = rh + rl with rh = float32(r):
-//     exp x = 2^n expf(rh) (1 + rl), about one float32 ulp (6e-8) relative.
+//   exp: x = (64 n + j) ln2 / 64 + b, |b| <= ln2 / 128, 2^(j/64) from a table:
+//     exp x = 2^n 2^(j/64) (1 + b + b^2 q), q in float32 (about 1e-12 relative).
 // Arguments outside the reductions' range (and NaN) take the float64 functions.
 #define SS_TRIG_H_HI 0.02454369260617026
 #define SS_TRIG_H_LO 9.567553118338697e-19
 #define SS_TRIG_INV_H 40.74366543152521
-#define SS_LN2_HI 0.6931471805599453
-#define SS_LN2_LO 2.3190468138462996e-17
-#define SS_INV_LN2 1.4426950408889634
+#define SS_EXP_H_HI 0.010830424696249145
+#define SS_EXP_H_LO 3.623510646634843e-19
+#define SS_EXP_INV_H 92.33248261689366
 
 // cos(2 pi j / 256), sin(2 pi j / 256), j = 0 .. 255
 __constant__ static const double ss_trig_table[512] = {
@@ -255,14 +255,43 @@ __device__ static inline double ss_sin32(double x)
     return fma(ca, fma(b, sm, b), fma(sa, cm, sa));
 }
 
+// 2^(j/64), j = 0 .. 63
+__constant__ static const double ss_exp_table[64] = {
+    1.0, 1.0108892860517005, 1.0218971486541166, 1.0330248790212284,
+    1.0442737824274138, 1.0556451783605572, 1.0671404006768237, 1.0787607977571199,
+    1.0905077326652577, 1.102382583307841, 1.1143867425958924, 1.1265216186082418,
+    1.1387886347566916, 1.1511892299529827, 1.1637248587775775, 1.1763969916502812,
+    1.189207115002721, 1.202156731452703, 1.215247359980469, 1.22848053610687,
+    1.241857812073484, 1.255380757024691, 1.2690509571917332, 1.2828700160787783,
+    1.2968395546510096, 1.3109612115247644, 1.3252366431597413, 1.339667524053303,
+    1.3542555469368927, 1.3690024229745905, 1.383909881963832, 1.3989796725383112,
+    1.4142135623730951, 1.42961333839197, 1.4451808069770467, 1.460917794180647,
+    1.4768261459394993, 1.4929077282912648, 1.5091644275934228, 1.5255981507445384,
+    1.5422108254079407, 1.559004400237837, 1.5759808451078865, 1.593142151342267,
+    1.6104903319492543, 1.6280274218573478, 1.645755478153965, 1.6636765803267364,
+    1.681792830507429, 1.7001063537185235, 1.718619298122478, 1.7373338352737062,
+    1.7562521603732995, 1.7753764925265212, 1.7947090750031072, 1.8142521755003989,
+    1.8340080864093424, 1.8539791250833855, 1.8741676341103, 1.8945759815869656,
+    1.9152065613971474, 1.9360617934922943, 1.9571441241754002, 1.978456026387951,
+};
+
+// exp x = 2^n 2^(j/64) exp(b): x = (64 n + j) ln2/64 + b, |b| <= ln2/128, and
+// exp(b) = 1 + b + b^2 q with q = 1/2 + b/6 + b^2/24 + b^3/120 in float32
+// (b^2 q < 1.5e-5: its rounding is about 1e-12 relative in the result).
+// Below -746 the float64 exponential is 0 exactly; beyond [-708, 709] (and
+// NaN) the float64 function is used.
 __device__ static inline double ss_exp32(double x)
 {
-    if (!(fabs(x) < 700.0)) return exp(x);
-    const double n = rint(x * SS_INV_LN2);
-    const double r = fma(-n, SS_LN2_LO, fma(-n, SS_LN2_HI, x));
-    const float rh = (float)r;
-    const double e = (double)expf(rh);
-    return ldexp(fma(e, r - (double)rh, e), (int)n);
+    if (x < -746.0) return 0.0;
+    if (!(x >= -708.0 && x <= 709.0)) return exp(x);
+    const double m = rint(x * SS_EXP_INV_H);
+    const double b = fma(-m, SS_EXP_H_LO, fma(-m, SS_EXP_H_HI, x));
+    const int mi = (int)m;
+    const double t = ss_exp_table[mi & 63];
+    const float bf = (float)b;
+    const float q = fmaf(bf, fmaf(bf, fmaf(bf, 1.0f / 120.0f, 1.0f / 24.0f), 1.0f / 6.0f), 0.5f);
+    const double em = b + (double)(bf * bf * q);
+    return ldexp(fma(t, em, t), mi >> 6);
 }
 
 #define SS_CONTEXT const ss_ctx* ss_ctx_ref
