@@ -353,17 +353,19 @@ def ensemble_oracle_states(sweep, dur, sims, psi0, cpu_cores):
     return refs, steps, cpu_s
 
 
-def batched_workload(ss, torch, precision, sweep, dur, refs, steps, warmup, n_sims=4096):
+def batched_workload(ss, torch, precision, sweep, dur, refs, steps, warmup, n_sims=4096,
+                     compact=False):
     """The ensemble in one persistent launch + batch state scan (Bloch vectors
     fused), replayed as a HIP graph, autotuned; steps/s and the state error of
-    the sampled simulations against the float64 oracle."""
+    the sampled simulations against the float64 oracle.  compact: operators
+    stored as first columns (32 B a row instead of 64)."""
     import numpy as np
 
     from spinsim_amd.autotune import tune_batch
 
     sim = ss.Simulator(RABI_SWEEP, ss.SpinQuantumNumber.HALF, precision=precision)
     psi0 = ss.SpinQuantumNumber.HALF.plus_z
-    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep, with_spin=True)
+    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep, with_spin=True, compact=compact)
     tune_batch(plan, reps=2, warmup=1, lanes=(1, 2, 4), items=(1, 4, 8, 16))
     plan.capture()
     for _ in range(warmup):
@@ -383,7 +385,8 @@ def batched_workload(ss, torch, precision, sweep, dur, refs, steps, warmup, n_si
            "lanes_per_step": plan.lanes, "items_per_grab": plan.items_per_grab,
            "scan_chunk_rows": plan.scan_chunk_rows,
            "state_err_vs_oracle_sampled": max(errs), "sampled_simulations": len(refs),
-           "precision": precision, "graph": True}
+           "precision": precision, "graph": True,
+           "operators": "first columns" if compact else "full"}
     del plan
     torch.cuda.empty_cache()
     return out
@@ -455,7 +458,12 @@ def ensemble_same_inputs(ss, torch, steps, cpu_cores, check=64):
     out = {"batched_fp32_bulk": batched_workload(ss, torch, "fp32_bulk", sweep, dur, refs, steps, 1),
            "batched_fp32": batched_workload(ss, torch, "fp32", sweep, dur, refs, steps, 1),
            "batched_fp64": batched_workload(ss, torch, "fp64", sweep, dur, refs, steps, 1),
-           "plain_fp64_loop": plain_loop_workload(ss, torch, sweep, dur, refs)}
+           "plain_fp64_loop": plain_loop_workload(ss, torch, sweep, dur, refs),
+           # states and Bloch vectors only: operators kept as first columns
+           "batched_fp32_compact": batched_workload(ss, torch, "fp32", sweep, dur, refs, steps, 1,
+                                                    compact=True),
+           "batched_fp64_compact": batched_workload(ss, torch, "fp64", sweep, dur, refs, steps, 1,
+                                                    compact=True)}
     plain = out["plain_fp64_loop"]["steps_per_s"]
     out["ratio_batched_fp32_bulk_over_plain_fp64"] = out["batched_fp32_bulk"]["steps_per_s"] / plain
     out["ratio_batched_fp64_over_plain_fp64"] = out["batched_fp64"]["steps_per_s"] / plain
@@ -464,6 +472,10 @@ def ensemble_same_inputs(ss, torch, steps, cpu_cores, check=64):
     out["ratio_batched_fp32_over_plain_fp64"] = out["batched_fp32"]["steps_per_s"] / plain
     out["ratio_fp32_over_fp32_bulk_batched"] = (out["batched_fp32"]["steps_per_s"]
                                                 / out["batched_fp32_bulk"]["steps_per_s"])
+    out["ratio_compact_over_full_fp32"] = (out["batched_fp32_compact"]["steps_per_s"]
+                                           / out["batched_fp32"]["steps_per_s"])
+    out["ratio_compact_over_full_fp64"] = (out["batched_fp64_compact"]["steps_per_s"]
+                                           / out["batched_fp64"]["steps_per_s"])
     out["target_2x_met"] = bool(out["ratio_batched_fp32_bulk_over_plain_fp64"] >= 2.0 and
                                 out["batched_fp32_bulk"]["state_err_vs_oracle_sampled"] <= 1e-6)
     if cpu_cores:

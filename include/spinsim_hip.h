@@ -99,6 +99,15 @@ enum { /* ExponentiationMethod, :95-129 */
                                  x, y, z, q, u1, u2, v1, v2) */
 };
 
+enum { /* layout of the batch's operator rows (no reference counterpart) */
+    SS_U_FULL = 0,      /* (total, dim, dim) complex128: every entry */
+    SS_U_SU2_COLUMN = 1 /* spin half only: (total, 2) complex128, the first column
+                           (a, b) of each U = [[a, -conj b], [b, conj a]] (the
+                           step operators of spin half are special unitary);
+                           half the bytes, for batches whose operators feed the
+                           state scan and are not read as matrices */
+};
+
 /* Message of the last failure on this thread ("" if none). */
 const char *ss_last_error(void);
 /* ABI version of this header (bumped on incompatible change). */
@@ -156,13 +165,15 @@ int ss_time_evolution_plan(int64_t n_coarse, int n_fine, int64_t waves, int *n_p
  * of time_out / u_out and reads sweep row s (sweep + s*sweep_stride).  All
  * arrays are device arrays; offsets has n_sims+1 entries, offsets[n_sims] =
  * total.  counter: one device uint64 the library clears before the launch.
- * grid_blocks = 0 picks a persistent grid for the device. */
+ * grid_blocks = 0 picks a persistent grid for the device.  u_layout: SS_U_FULL,
+ * or SS_U_SU2_COLUMN for a spin-half module (u_out then holds total*2
+ * complex128). */
 int ss_batch_time_evolution(const ss_module *m, const double *sweep, int64_t sweep_stride,
                             int64_t n_sims, const int64_t *offsets, const double *t0,
                             const double *dt, const double *dt_out, const int32_t *n_fine,
                             int64_t total, double *time_out, double *u_out,
                             const ss_table *tables, uint64_t *counter, int lanes_per_step,
-                            int items_per_grab, int grid_blocks, ss_stream stream);
+                            int items_per_grab, int grid_blocks, int u_layout, ss_stream stream);
 
 /* Workspace bytes ss_state_scan (n rows) and ss_batch_state_scan /
  * ss_batch_costate_scan with the default chunking (chunk_rows 0; n = total
@@ -186,11 +197,12 @@ int ss_state_scan(const double *psi0, const double *u, double *state, double *sp
 /* Per-simulation state scan over CSR rows (total = offsets[n_sims]); psi0:
  * device (n_sims, dim); spin: NULL or device (total, 3).  chunk_rows: rows per
  * workgroup chunk, 0 for the default or a positive multiple of 1024; the
- * workspace holds ss_batch_scan_workspace(total, dim, chunk_rows) bytes. */
+ * workspace holds ss_batch_scan_workspace(total, dim, chunk_rows) bytes.
+ * u_layout: how u is stored (SS_U_FULL; SS_U_SU2_COLUMN with dim 2). */
 int ss_batch_state_scan(const double *psi0, const double *u, double *state, double *spin,
                         const int64_t *offsets, int64_t n_sims, int64_t total, int dim,
-                        int64_t chunk_rows, void *workspace, size_t workspace_bytes,
-                        ss_stream stream);
+                        int64_t chunk_rows, int u_layout, void *workspace,
+                        size_t workspace_bytes, ss_stream stream);
 
 /* ss_batch_state_scan without a workspace argument and without allocating:
  * the chunk records live in a static device array of the library (416 KiB

@@ -40,14 +40,14 @@ SIGNATURES = {
                                   c_int, c_vp]),
     "ss_batch_time_evolution": (c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
                                         c_i64, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int,
-                                        c_vp]),
+                                        c_int, c_vp]),
     "ss_state_scan_workspace": (ctypes.c_size_t, [c_i64, c_int]),
     "ss_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, ctypes.c_size_t,
                               c_vp]),
     "ss_batch_scan_workspace": (ctypes.c_size_t, [c_i64, c_int, c_i64]),
     "ss_time_evolution_plan": (c_int, [c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "ss_batch_state_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
-                                    c_vp, ctypes.c_size_t, c_vp]),
+                                    c_int, c_vp, ctypes.c_size_t, c_vp]),
     "ss_batch_state_scan_static": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int,
                                            c_i64, c_vp]),
     "ss_batch_costate_scan": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_i64,
@@ -64,7 +64,11 @@ SIGNATURES = {
 
 
 # Version of the C ABI these signatures describe (ss_abi_version()).
-ABI_VERSION = 8
+ABI_VERSION = 9
+
+# Operator row layouts of the batch entry points (SS_U_* of the C header).
+U_FULL = 0
+U_SU2_COLUMN = 1
 
 
 class NativeError(RuntimeError):

@@ -59,16 +59,23 @@ class BatchResults:
     ``time`` (total,), ``time_evolution`` (total, d, d), ``state`` (total, d),
     lazy ``spin`` (total, 3); rows of simulation s are
     ``offsets[s]:offsets[s+1]``.  ``results[s]`` is that simulation's
-    :class:`Results` (numpy).
+    :class:`Results` (numpy).  A compact plan's ``time_evolution`` is expanded
+    from the stored first columns when first read.
     """
 
     def __init__(self, plan):
         self._plan = plan
         self.offsets = plan.offsets_host
         self.time = plan.time
-        self.time_evolution = plan.u
         self.state = plan.state
+        self._u = None if plan.compact else plan.u
         self._spin = None
+
+    @property
+    def time_evolution(self):
+        if self._u is None:
+            self._u = su2_from_columns(self._plan.u)
+        return self._u
 
     @property
     def spin(self):
@@ -91,20 +98,35 @@ class BatchResults:
 SCAN_TILE_ROWS = 1024
 
 
+def su2_from_columns(col):
+    """(n, 2) first columns (a, b) -> (n, 2, 2) U = [[a, -conj b], [b, conj a]]."""
+    import torch
+
+    a, b = col[:, 0], col[:, 1]
+    return torch.stack([a, -b.conj(), b, a.conj()], 1).resolve_conj().resolve_neg().view(-1, 2, 2)
+
+
 class BatchPlan:
     """Device buffers and launch parameters for one batched workload.
 
     ``run()`` enqueues time evolution + state scan (+ Bloch vectors into
     ``spin_out`` when ``with_spin``) on the current stream;
     ``capture()`` records them into a HIP graph and ``replay()`` re-runs it.
+    ``compact=True`` (spin half) stores each operator as its first column,
+    ``u`` (total, 2): half the operator bytes written and read back by the
+    scan, for workloads that want states and Bloch vectors.
     """
 
     def __init__(self, sim, time_start, time_end, time_step_integration, time_step_output,
                  state_init, sweep_parameters, *, lanes_per_step=None, items_per_grab=None,
-                 grid_blocks=0, with_spin=False):
+                 grid_blocks=0, with_spin=False, compact=False):
         torch = sim._torch()
         self.sim = sim
         d = sim.dimension
+        if compact and d != 2:
+            raise ValueError("compact operator storage is for spin half (dimension 2)")
+        self.compact = bool(compact)
+        self.u_layout = _native.U_SU2_COLUMN if compact else _native.U_FULL
         sweep = np.asarray(sweep_parameters, np.float64)
         if sweep.ndim == 1:
             sweep = sweep[None, :]
@@ -140,7 +162,8 @@ class BatchPlan:
         self.psi0 = torch.as_tensor(np.ascontiguousarray(psi), device=dev)
         self.counter = torch.zeros(1, dtype=torch.int64, device=dev)
         self.time = torch.empty(total, dtype=f64, device=dev)
-        self.u = torch.empty((total, d, d), dtype=torch.complex128, device=dev)
+        self.u = torch.empty((total, 2) if compact else (total, d, d), dtype=torch.complex128,
+                             device=dev)
         self.state = torch.empty((total, d), dtype=torch.complex128, device=dev)
         self.lanes = int(lanes_per_step or sim.lanes_per_step or 0)
         if self.lanes <= 0:
@@ -186,7 +209,7 @@ class BatchPlan:
             self.offsets.data_ptr(), self.t0.data_ptr(), self.dt.data_ptr(),
             self.dt_out.data_ptr(), self.n_fine.data_ptr(), self.total, self.time.data_ptr(),
             self.u.data_ptr(), self.sim._table_ptr(self.device), self.counter.data_ptr(),
-            self.lanes, self.items_per_grab, self.grid_blocks, stream),
+            self.lanes, self.items_per_grab, self.grid_blocks, self.u_layout, stream),
             "ss_batch_time_evolution")
 
     def run_state_scan(self, stream=None):
@@ -197,8 +220,8 @@ class BatchPlan:
         _native.check(_native.lib().ss_batch_state_scan(
             self.psi0.data_ptr(), self.u.data_ptr(), self.state.data_ptr(),
             None if spin is None else spin.data_ptr(), self.offsets.data_ptr(), self.n_sims,
-            self.total, self.sim.dimension, self.scan_chunk_rows, self.scan_ws.data_ptr(),
-            self.scan_ws.numel(), stream),
+            self.total, self.sim.dimension, self.scan_chunk_rows, self.u_layout,
+            self.scan_ws.data_ptr(), self.scan_ws.numel(), stream),
             "ss_batch_state_scan")
 
     def run(self):

@@ -43,7 +43,7 @@ struct ss_batch_args {
     const double* dt_out;        // (B,)
     const int* n_fine;           // (B,)
     double* time_out;            // (total,)
-    double* u_out;               // (total, d, d) complex128
+    double* u_out;               // (total, d, d) complex128, or (total, 2) with u_layout 1
     unsigned long long* counter; // work counter, zero before launch
     long long total;             // offsets[B]
     long long n_sims;            // B
@@ -52,6 +52,8 @@ struct ss_batch_args {
     double off_b_frac;
     int lanes_per_step;
     int items_per_grab;
+    int u_layout;                // 0: full operators; 1: spin half, first column only
+    int pad_;
 };
 
 // One control: field[axis] += scale * (interpolation table `table`)(t), whose

@@ -920,9 +920,13 @@ static __device__ __forceinline__ void ss_reunitarise(ss_cd u[SS_D][SS_D])
 #endif
 
 // Leave the rotating frame and store U and time, :838-852, :818.  Float32-bulk
-// operators are re-unitarised first (once per coarse step).
+// operators are re-unitarised first (once per coarse step).  layout 1 (spin
+// half, batches whose operators only feed the state scan) stores the first
+// column (U00, U10) alone: a spin-half step operator is special unitary,
+// U = [[a, -conj b], [b, conj a]], and stays so through the frame exit, so
+// the column determines it.
 static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot, double dt_out, double tc,
-                                                long long row, double* time_out, double* u_out)
+                                                long long row, double* time_out, double* u_out, int layout = 0)
 {
 #if SS_PREC != 0
     ss_reunitarise(u);
@@ -937,6 +941,17 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
         u[0][k] = ss_mul(u[0][k], wc);
         u[SS_D - 1][k] = ss_mul(u[SS_D - 1][k], w);
     }
+#endif
+#if SS_DIM == 2
+    if (layout == 1) {
+        double2* col = reinterpret_cast<double2*>(u_out) + row * 2;
+        col[0] = make_double2(u[0][0].x, u[0][0].y);
+        col[1] = make_double2(u[1][0].x, u[1][0].y);
+        time_out[row] = tc;
+        return;
+    }
+#else
+    (void)layout;
 #endif
     double2* dst = reinterpret_cast<double2*>(u_out) + row * (SS_D * SS_D);
 #pragma unroll
@@ -1060,7 +1075,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
             ss_cd u[SS_D][SS_D];
             ss_integrate(u, j0, count, tc, rot, dt, half * a.off_a_frac, half * a.off_b_frac, sweep, &ctx);
             ss_group_reduce(u, rank, lanes);
-            if (active && rank == 0) ss_store(u, rot, dt_out, tc, row, a.time_out, a.u_out);
+            if (active && rank == 0) ss_store(u, rot, dt_out, tc, row, a.time_out, a.u_out, a.u_layout);
         }
     }
 }

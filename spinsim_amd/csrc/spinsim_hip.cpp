@@ -50,7 +50,7 @@ int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-constexpr int kAbiVersion = 8;
+constexpr int kAbiVersion = 9;
 static_assert(sizeof(::ss_table) == 56, "ss_table layout");
 static_assert(sizeof(::ss_table) == sizeof(dev::ss_table), "ss_table layout");
 static_assert(sizeof(::ss_control_desc) == sizeof(ss_control), "ss_control_desc layout");
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(kSingleScanBlock) single_scan_apply_carry(cons
 constexpr int kScanThreads = 256;
 
 struct ScanArgs {
-    const double* u;            // (total, D, D)
+    const double* u;            // (total, D, D), or (total, 2) first columns with u_layout 1
     double* out;                // (total, D) states, or costates
     double* spin;               // (total, 3) or null
     const long long* offsets;   // (n_sims + 1,) CSR rows, or null: one simulation
@@ -381,6 +381,7 @@ struct ScanArgs {
     long long chunk_rows;       // a multiple of kScanThreads
     long long n_chunks;
     double* agg;                // (n_chunks, kAggStride(D)): m, c, constant flag
+    int u_layout;               // SS_U_FULL or SS_U_SU2_COLUMN (spin half, forward)
 };
 
 // Doubles per chunk record (m, c, flag), padded to whole 16-byte pairs.
@@ -434,10 +435,24 @@ __device__ __forceinline__ void load_vec_c(const double* p, ss_cd v[D])
 
 // Matrix leading into scan-order row r (r >= 1): U[r - 1] forward,
 // U[total - r]^+ backward (costate[i] = U[i+1]^+ costate[i+1]).
+// Spin-half operator k from its first column (a, b): U = [[a, -conj b], [b, conj a]].
+template <int D>
+__device__ __forceinline__ void load_su2_column(const double* u, long long k, ss_cd m[D][D])
+{
+    const double2* p = reinterpret_cast<const double2*>(u) + k * 2;
+    const double2 c0 = p[0], c1 = p[1];
+    m[0][0] = ss_c(c0.x, c0.y);
+    m[1][0] = ss_c(c1.x, c1.y);
+    m[0][1] = ss_c(-c1.x, c1.y);
+    m[1][1] = ss_c(c0.x, -c0.y);
+}
+
 template <int D, bool REV>
 __device__ __forceinline__ void scan_row_mat(const ScanArgs& a, long long r, ss_cd m[D][D])
 {
-    if (!REV) {
+    if (D == 2 && !REV && a.u_layout == SS_U_SU2_COLUMN) {
+        load_su2_column<D>(a.u, r - 1, m);
+    } else if (!REV) {
         load_mat<D>(a.u, r - 1, m);
     } else {
         ss_cd t[D][D];
@@ -1351,9 +1366,12 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
                             const int64_t* offsets, const double* t0, const double* dt, const double* dt_out,
                             const int32_t* n_fine, int64_t total, double* time_out, double* u_out,
                             const ss_table* tables, uint64_t* counter, int lanes_per_step, int items_per_grab,
-                            int grid_blocks, ss_stream stream)
+                            int grid_blocks, int u_layout, ss_stream stream)
 {
     if (!m) return fail("ss_batch_time_evolution: null module");
+    if (u_layout != SS_U_FULL && !(u_layout == SS_U_SU2_COLUMN && m->dim == 2))
+        return fail("ss_batch_time_evolution: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN for spin half)",
+                    u_layout);
     if (n_sims < 0 || total < 0) return fail("ss_batch_time_evolution: negative size");
     if (n_sims == 0 || total == 0) return 0;
     if (!sweep || !offsets || !t0 || !dt || !dt_out || !n_fine || !time_out || !u_out || !counter)
@@ -1396,6 +1414,8 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
     a.off_b_frac = ob;
     a.lanes_per_step = lanes_per_step;
     a.items_per_grab = items_per_grab;
+    a.u_layout = u_layout;
+    a.pad_ = 0;
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, s, a.counter);
     SS_HIP(hipGetLastError());
     void* params[] = {&a};
@@ -1537,10 +1557,12 @@ int ss_state_scan(const double* psi0, const double* u, double* state, double* sp
 }
 
 int ss_batch_state_scan(const double* psi0, const double* u, double* state, double* spin, const int64_t* offsets,
-                        int64_t n_sims, int64_t total, int dim, int64_t chunk_rows, void* workspace,
-                        size_t workspace_bytes, ss_stream stream)
+                        int64_t n_sims, int64_t total, int dim, int64_t chunk_rows, int u_layout,
+                        void* workspace, size_t workspace_bytes, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_state_scan: dim must be 2 or 3");
+    if (u_layout != SS_U_FULL && !(u_layout == SS_U_SU2_COLUMN && dim == 2))
+        return fail("ss_batch_state_scan: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN with dim 2)", u_layout);
     if (n_sims < 0 || total < 0) return fail("ss_batch_state_scan: negative size");
     if (n_sims == 0 || total == 0) return 0;
     if (!psi0 || !u || !state || !offsets) return fail("ss_batch_state_scan: null buffer");
@@ -1554,6 +1576,7 @@ int ss_batch_state_scan(const double* psi0, const double* u, double* state, doub
     a.init = psi0;
     a.total = total;
     a.n_sims = n_sims;
+    a.u_layout = u_layout;
     return launch_scan("ss_batch_state_scan", a, dim, false, chunk_rows, workspace, workspace_bytes,
                         static_cast<hipStream_t>(stream));
 }

@@ -61,7 +61,7 @@ def _run(u, offsets, init, reverse, with_spin=False, chunk_rows=0):
     else:
         rc = L.ss_batch_state_scan(dinit.data_ptr(), du.data_ptr(), out.data_ptr(),
                                    None if spin is None else spin.data_ptr(), doff.data_ptr(), b,
-                                   total, d, chunk_rows, ws.data_ptr(), ws.numel(), stream)
+                                   total, d, chunk_rows, 0, ws.data_ptr(), ws.numel(), stream)
     _native.check(rc, "scan")
     return out.cpu().numpy(), (None if spin is None else spin.cpu().numpy())
 
@@ -188,8 +188,8 @@ def test_batch_scan_graph_replays():
     def run():
         s = torch.cuda.current_stream().cuda_stream
         _native.check(L.ss_batch_state_scan(dinit.data_ptr(), du.data_ptr(), out.data_ptr(), None,
-                                            doff.data_ptr(), len(n), total, d, 0, ws.data_ptr(),
-                                            ws.numel(), s), "scan")
+                                            doff.data_ptr(), len(n), total, d, 0, 0,
+                                            ws.data_ptr(), ws.numel(), s), "scan")
 
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.ss_abi_version() == _native.ABI_VERSION == 8
+    assert lib.ss_abi_version() == _native.ABI_VERSION == 9
     assert lib.ss_source_hash() != 0
 
 
@@ -49,13 +49,18 @@ def test_library_rejects_bad_arguments():
     assert lib.ss_spin(None, None, -1, 2, None) != 0
     assert lib.ss_state_scan_workspace(0, 2) == 0
     assert lib.ss_state_scan_workspace(100, 3) > 0
-    assert lib.ss_batch_state_scan(None, None, None, None, None, 3, 10, 2, 0, None, 0, None) != 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 3, 10, 2, 0, 0, None, 0, None) != 0
     assert b"null buffer" in lib.ss_last_error()
     assert lib.ss_time_evolution(None, None, 0.0, 1e-7, 1e-6, 10, None, None, None, 65, None) != 0
     assert lib.ss_fma_probe(None, 0, 1, None) != 0
     # zero-sized work is a no-op success, even with null buffers
     assert lib.ss_spin(None, None, 0, 2, None) == 0
-    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, None, 0, None) == 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 0, None, 0, None) == 0
+    # the first-column operator layout is for spin half only
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 3, 0, 1, None, 0, None) != 0
+    assert b"u_layout" in lib.ss_last_error()
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 2, None, 0, None) != 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 1, None, 0, None) == 0
 
 
 def test_batch_scan_chunk_sizes():
@@ -70,7 +75,7 @@ def test_batch_scan_chunk_sizes():
     # a chunk size that is not a whole number of tiles is rejected before launch
     p = ctypes.c_void_p(256)
     for bad in (1000, -1024, 1536):
-        assert lib.ss_batch_state_scan(p, p, p, None, p, 1, 5000, 2, bad, p, 1 << 20, None) != 0
+        assert lib.ss_batch_state_scan(p, p, p, None, p, 1, 5000, 2, bad, 0, p, 1 << 20, None) != 0
         assert b"chunk_rows" in lib.ss_last_error()
         assert lib.ss_batch_costate_scan(p, p, p, p, 1, 5000, 3, bad, p, 1 << 20, None) != 0
         assert b"chunk_rows" in lib.ss_last_error()

@@ -47,7 +47,7 @@ def _scan_both(torch, L, d, offsets, u, psi, chunk_rows, stream):
             ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
             rc = L.ss_batch_state_scan(psi_d.data_ptr(), u_d.data_ptr(), state.data_ptr(),
                                        spin.data_ptr(), off.data_ptr(), n_sims, total, d,
-                                       chunk_rows, ws.data_ptr(), nbytes, stream)
+                                       chunk_rows, 0, ws.data_ptr(), nbytes, stream)
         else:
             rc = L.ss_batch_state_scan_static(psi_d.data_ptr(), u_d.data_ptr(), state.data_ptr(),
                                               spin.data_ptr(), off.data_ptr(), n_sims, total, d,
