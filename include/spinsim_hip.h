@@ -119,7 +119,11 @@ uint64_t ss_source_hash(void);
  * needed.  `field_src` defines
  *   __device__ void get_field(double t, const double *sweep, double *field)
  * or, with field_has_context = 1, the same with a trailing SS_CONTEXT
- * argument (for samplers).  dim is 2 (spin half) or 3 (spin one);
+ * argument (for samplers).  The text may also define SS_ADJ_CONTROLS (1..4,
+ * the number of controls ss_batch_adjoint will be given: per-lane state for
+ * that many; default 4), SS_WAVES_PER_EU or SS_ADJ_WAVES_PER_EU (the waves
+ * per SIMD the time-evolution and adjoint kernels are compiled for; defaults
+ * 1 and 2).  dim is 2 (spin half) or 3 (spin one);
  * number_of_squares as in the reference (Lie-Trotter squarings, :1547-1550);
  * precision: SS_PRECISION_*.
  * On success *code / *code_size hold a code object and *log the compiler log

@@ -56,8 +56,10 @@ class SsControlDesc(ctypes.Structure):
 
 
 def _combined_source(static, controls):
-    """HIP-C text: the static field (renamed) plus the control terms."""
-    parts = []
+    """HIP-C text: the static field (renamed) plus the control terms; it sets
+    SS_ADJ_CONTROLS, the adjoint kernel's number of per-lane gradient moment
+    sums, to the control count."""
+    parts = [f"#define SS_ADJ_CONTROLS {len(controls)}\n"]
     if static is not None:
         parts.append("#define get_field ss_static_field\n" + static.source + "\n#undef get_field\n")
     call = ""

@@ -67,10 +67,13 @@ struct ss_control {
 
 #define SS_MAX_CONTROLS 4
 
-// Adjoint workgroup size, and the gradient taps each lane keeps per control
-// in LDS: the launch gives the kernel n_controls * SS_ADJ_WINDOW *
+// Adjoint workgroup size, and the LDS slots each lane keeps per control:
+// SS_ADJ_WINDOW gradient taps (knots) and SS_ADJ_MOMENTS moment sums of the
+// current interval; the launch gives the kernel n_controls * SS_ADJ_SLOTS *
 // SS_ADJ_BLOCK doubles of dynamic shared memory.
 #define SS_ADJ_WINDOW 8
+#define SS_ADJ_MOMENTS 4
+#define SS_ADJ_SLOTS (SS_ADJ_WINDOW + SS_ADJ_MOMENTS)
 #define SS_ADJ_BLOCK 256
 
 // Batched adjoint launch: one lane per coarse row.  state[row] is the state at

@@ -19,6 +19,7 @@ static __device__ __forceinline__ ss_cd ss_mul(ss_cd a, ss_cd b)
 }
 static __device__ __forceinline__ ss_cd ss_conj(ss_cd a) { return ss_c(a.x, -a.y); }
 static __device__ __forceinline__ ss_cd ss_add(ss_cd a, ss_cd b) { return ss_c(a.x + b.x, a.y + b.y); }
+static __device__ __forceinline__ ss_cd ss_sub(ss_cd a, ss_cd b) { return ss_c(a.x - b.x, a.y - b.y); }
 static __device__ __forceinline__ ss_cd ss_scale(ss_cd a, double s) { return ss_c(a.x * s, a.y * s); }
 // acc + a*b as four fused multiply-adds
 static __device__ __forceinline__ ss_cd ss_mac(ss_cd acc, ss_cd a, ss_cd b)

@@ -963,6 +963,21 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
 
 // ---- GPU kernels (the host build of this text, for the CPU devices, stops here)
 #ifndef SS_HOST
+// Waves per SIMD the kernels are compiled for at least: the register budget
+// is 512 / waves VGPRs a lane (in 8-register granules).  The time-evolution
+// kernels take what they need (about 230-260 for spin one: their float64
+// chains keep one or two waves per SIMD busy); the adjoint kernel is held to
+// 256, two waves per SIMD, to hide its table and LDS latencies (8.4 ms
+// against 11.3 ms at 270 VGPRs and one wave, for 1024 spin-one pulses of
+// 1e5 steps, at the cost of a few spilled registers).  A field source may
+// define either to retune.
+#ifndef SS_WAVES_PER_EU
+#define SS_WAVES_PER_EU 1
+#endif
+#ifndef SS_ADJ_WAVES_PER_EU
+#define SS_ADJ_WAVES_PER_EU 2
+#endif
+#define SS_KERNEL_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SS_WAVES_PER_EU)))
 // One item of a single-simulation launch (get_time_evolution,
 // spinsim/__init__.py:854-896): in phase p, whose items start at item number
 // `first`, group gi of `lanes` lanes integrates coarse row
@@ -992,7 +1007,7 @@ static __device__ __forceinline__ void ss_te_item(const ss_te_args& a, long long
 
 // One simulation, one item per wave (a single phase: phase_lanes[0] lanes per
 // coarse step for every row).
-extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te_args a)
+extern "C" __global__ void SS_KERNEL_ATTR ss_time_evolution_kernel(ss_te_args a)
 {
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (wave >= a.items) return;  // whole waves only
@@ -1006,7 +1021,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_kernel(ss_te
 // w + 2 waves, ... of the phase table.  Phases with fewer lanes per row (long
 // items) come first and the finest last, so every wave ends within about one
 // fine step of the others.
-extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_phased_kernel(ss_te_args a)
+extern "C" __global__ void SS_KERNEL_ATTR ss_time_evolution_phased_kernel(ss_te_args a)
 {
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     ss_ctx ctx;
@@ -1028,7 +1043,7 @@ extern "C" __global__ void __launch_bounds__(256) ss_time_evolution_phased_kerne
 // `items_per_grab` groups' worth of coarse rows from a global counter (one
 // atomic per wave), so simulations with 50x different lengths finish
 // together; every wave leaves once the counter passes the total.
-extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel(ss_batch_args a)
+extern "C" __global__ void SS_KERNEL_ATTR ss_batch_time_evolution_kernel(ss_batch_args a)
 {
     const int lane = threadIdx.x & 63;
     const int lanes = a.lanes_per_step;
@@ -1105,10 +1120,13 @@ extern "C" __global__ void __launch_bounds__(256) ss_batch_time_evolution_kernel
 // f the transformed sample, psi and lam at the interval's end; the sum is
 // scattered onto the z controls at the midpoint.
 // ===========================================================================
-#define SS_ADJ_ORDER 5       // Frechet series order (terms up to n = 5)
-#define SS_ADJ_TAYLOR 8      // exp(-A) v Taylor order
-#define SS_ADJ_NU 0.05       // sub-step until the generator norm bound is below this
-#define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 51)
+#ifndef SS_ADJ_ORDER
+#define SS_ADJ_ORDER 5
+#endif
+//       // Frechet series: terms p + q <= 5
+#define SS_ADJ_TAYLOR 7      // Taylor order of the exponentials walking psi and lambda back
+#define SS_ADJ_NU 0.035      // sub-step until the generator norm bound is below this
+#define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 35.8)
 
 struct ss_gen64 {
     double d[SS_D];
@@ -1137,92 +1155,133 @@ static __device__ __forceinline__ ss_gen64 ss_generator64(const double* g, doubl
     return a;
 }
 
-// out = s A v  (s = +-1 or a scale)
-static __device__ __forceinline__ void ss_gen_vec(const ss_gen64& a, double s, const ss_cd v[SS_D], ss_cd out[SS_D])
+// out = A v (A tridiagonal: 6 multiplications and 16 fused multiply-adds for spin one)
+static __device__ __forceinline__ void ss_gen_vec(const ss_gen64& a, const ss_cd v[SS_D], ss_cd out[SS_D])
 {
 #pragma unroll
     for (int r = 0; r < SS_D; ++r) {
         ss_cd acc = ss_c(-a.d[r] * v[r].y, a.d[r] * v[r].x);
         if (r > 0) acc = ss_mac(acc, a.l, v[r - 1]);
         if (r + 1 < SS_D) acc = ss_mac(acc, a.u, v[r + 1]);
-        out[r] = ss_scale(acc, s);
+        out[r] = acc;
     }
 }
 
-// v <- exp(-A) v by its Taylor series (|A| <= SS_ADJ_NU: truncation below
-// nu^9 / 9! = 5.4e-18 relative).
-static __device__ __forceinline__ void ss_expm_neg_vec(const ss_gen64& a, ss_cd v[SS_D])
+// acc += c v
+static __device__ __forceinline__ void ss_axpy(ss_cd acc[SS_D], double c, const ss_cd v[SS_D])
 {
-    ss_cd term[SS_D], acc[SS_D];
 #pragma unroll
-    for (int r = 0; r < SS_D; ++r) {
-        term[r] = v[r];
-        acc[r] = v[r];
+    for (int r = 0; r < SS_D; ++r) acc[r] = ss_c(fma(c, v[r].x, acc[r].x), fma(c, v[r].y, acc[r].y));
+}
+
+// 1/n!, n = 0..SS_ADJ_TAYLOR
+static constexpr double ss_inv_fact[8] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120,
+                                                     1.0 / 720, 1.0 / 5040};
+
+// grad[k] += w Re(gam^+ E_k b) for the axes in mask, E_k = ss_generator64(e_k, 1):
+// the generators' few nonzero entries written out (spin one: x and y couple
+// neighbouring components with -i/sqrt2 and -+1/sqrt2, z and q are the
+// diagonals i(-1, 0, 1) and i(-1, 2, -1)/3; spin half: the Pauli matrices
+// times -i/2).
+static __device__ __forceinline__ void ss_grad_dots(const ss_cd gam[SS_D], const ss_cd b[SS_D], unsigned mask, double w,
+                                                    double grad[SS_LIE])
+{
+#if SS_DIM == 2
+    const double h = 0.5 * w;
+    if (mask & 1u)
+        grad[0] += h * (gam[0].x * b[1].y - gam[0].y * b[1].x + gam[1].x * b[0].y - gam[1].y * b[0].x);
+    if (mask & 2u)
+        grad[1] += h * (gam[1].x * b[0].x + gam[1].y * b[0].y - gam[0].x * b[1].x - gam[0].y * b[1].y);
+    if (mask & 4u)
+        grad[2] += h * (gam[0].x * b[0].y - gam[0].y * b[0].x - gam[1].x * b[1].y + gam[1].y * b[1].x);
+#else
+    const double c = w * SS_INV_SQRT2;
+    if (mask & 1u) {
+        const ss_cd s1 = ss_add(gam[0], gam[2]), s2 = ss_add(b[0], b[2]);
+        grad[0] += c * (s1.x * b[1].y - s1.y * b[1].x + gam[1].x * s2.y - gam[1].y * s2.x);
     }
+    if (mask & 2u) {
+        const ss_cd d1 = ss_sub(gam[2], gam[0]), d2 = ss_sub(b[0], b[2]);
+        grad[1] += c * (d1.x * b[1].x + d1.y * b[1].y + gam[1].x * d2.x + gam[1].y * d2.y);
+    }
+    if (mask & 12u) {
+        // Im(conj(gam_r) b_r) per component
+        const double i0 = gam[0].x * b[0].y - gam[0].y * b[0].x;
+        const double i1 = gam[1].x * b[1].y - gam[1].y * b[1].x;
+        const double i2 = gam[2].x * b[2].y - gam[2].y * b[2].x;
+        if (mask & 4u) grad[2] += w * (i0 - i2);
+        if (mask & 8u) grad[3] += w * (1.0 / 3.0) * (i0 - 2.0 * i1 + i2);
+    }
+#endif
+}
+
+// One piece walked backward: psi <- e^{-A} psi, lambda <- e^{-A} lambda
+// (Taylor series to order SS_ADJ_TAYLOR: below nu^8 / 8! = 5.6e-17 relative
+// at the sub-step bound), and grad[k] += w Re(lambda^+ L(A, E_k) psi_before)
+// for the axes in mask, L(A, E) = sum_{p,q} A^p E A^q / (p + q + 1)!:
+//     Re(lambda^+ L(A, E) psi_before) = sum_{p+q<=N} c_{p+q} Re(alpha_p^+ E beta_q),
+// alpha_p = (A^+)^p lambda = (-A)^p lambda (kept: they are also the terms of
+// lambda's own series), beta_q = A^q psi_before (streamed), c_n = 1/(n+1)!,
+// and gamma_q = sum_p c_{p+q} alpha_p formed for each q.  The series left
+// out (p + q > N = SS_ADJ_ORDER) is below nu^(N+1) / (N+1)! of |E|
+// relative: 2.5e-12 at nu = 0.035, 6.7e-15 at the bench workload's 0.013.
+static __device__ __forceinline__ void ss_adjoint_piece(const ss_gen64& na, ss_cd psi[SS_D], ss_cd lam[SS_D],
+                                                        unsigned mask, double w, double grad[SS_LIE])
+{
+    // psi <- e^{-A} psi = sum_q na^q psi / q!
+    {
+        ss_cd t[SS_D], t2[SS_D], acc[SS_D];
 #pragma unroll
-    for (int n = 1; n <= SS_ADJ_TAYLOR; ++n) {
-        ss_cd t[SS_D];
-        ss_gen_vec(a, -1.0 / n, term, t);
+        for (int r = 0; r < SS_D; ++r) t[r] = acc[r] = psi[r];
 #pragma unroll
-        for (int r = 0; r < SS_D; ++r) {
-            term[r] = t[r];
-            acc[r] = ss_add(acc[r], t[r]);
+        for (int q = 1; q <= SS_ADJ_TAYLOR; ++q) {
+            ss_gen_vec(na, t, t2);
+#pragma unroll
+            for (int r = 0; r < SS_D; ++r) t[r] = t2[r];
+            ss_axpy(acc, ss_inv_fact[q], t);
         }
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) psi[r] = acc[r];
     }
+    // alpha_p = na^p lambda
+    ss_cd al[SS_ADJ_ORDER + 1][SS_D];
 #pragma unroll
-    for (int r = 0; r < SS_D; ++r) v[r] = acc[r];
-}
-
-static __device__ __forceinline__ double ss_re_dot(const ss_cd a[SS_D], const ss_cd b[SS_D])
-{
-    double s = 0.0;
+    for (int r = 0; r < SS_D; ++r) al[0][r] = lam[r];
 #pragma unroll
-    for (int r = 0; r < SS_D; ++r) s += a[r].x * b[r].x + a[r].y * b[r].y;  // Re(a^+ b)
-    return s;
-}
-
-// Accumulate grad[k] += w * Re(lam^+ L(A, A(e_k)) psi) for the axes in mask:
-//     Re(lam^+ L(A, E) psi) = sum_{p+q<=N} c_{p+q} Re(alpha_p^+ E beta_q),
-// alpha_p = (A^+)^p lam = (-A)^p lam, beta_q = A^q psi, c_n = 1/(n+1)!; both
-// power sequences once (2N generator products), then for each q the
-// combination gamma_q = sum_p c_{p+q} alpha_p (vector sums only).  The
-// series left out is below nu^(N+1) / (N+1)! of |E| relative: 2.2e-11 at the
-// sub-step bound nu = 0.05, 2e-21 at the 1e-3 of a 20 ns step in a 2 pi 5 kHz
-// field.
-static __device__ __forceinline__ void ss_frechet_grad(const ss_gen64& a, const ss_cd lam[SS_D],
-                                                       const ss_cd psi[SS_D], unsigned mask, double w,
-                                                       double grad[SS_LIE])
-{
-    ss_cd beta[SS_ADJ_ORDER + 1][SS_D], alpha[SS_ADJ_ORDER + 1][SS_D];
+    for (int p = 1; p <= SS_ADJ_ORDER; ++p) ss_gen_vec(na, al[p - 1], al[p]);
+    // beta_q = A^q psi_before = (-1)^q na^q psi, streamed
+    ss_cd b[SS_D];
 #pragma unroll
-    for (int r = 0; r < SS_D; ++r) {
-        beta[0][r] = psi[r];
-        alpha[0][r] = lam[r];
-    }
-#pragma unroll
-    for (int q = 1; q <= SS_ADJ_ORDER; ++q) {
-        ss_gen_vec(a, 1.0, beta[q - 1], beta[q]);
-        ss_gen_vec(a, -1.0, alpha[q - 1], alpha[q]);
-    }
-    const double c[7] = {1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    for (int r = 0; r < SS_D; ++r) b[r] = psi[r];
 #pragma unroll
     for (int q = 0; q <= SS_ADJ_ORDER; ++q) {
+        if (q > 0) {
+            ss_cd t[SS_D];
+            ss_gen_vec(na, b, t);
+#pragma unroll
+            for (int r = 0; r < SS_D; ++r) b[r] = t[r];
+        }
+        const double sign = (q & 1) ? -1.0 : 1.0;
         ss_cd gam[SS_D];
 #pragma unroll
-        for (int r = 0; r < SS_D; ++r) gam[r] = ss_scale(alpha[0][r], c[q]);
+        for (int r = 0; r < SS_D; ++r) gam[r] = ss_scale(al[0][r], sign * ss_inv_fact[q + 1]);
 #pragma unroll
-        for (int p = 1; p <= SS_ADJ_ORDER - q; ++p)
+        for (int p = 1; p <= SS_ADJ_ORDER - q; ++p) ss_axpy(gam, sign * ss_inv_fact[p + q + 1], al[p]);
+        ss_grad_dots(gam, b, mask, w, grad);
+    }
+    // lambda <- sum_p alpha_p / p!, the orders past N continued from alpha_N
+    {
+        ss_cd t[SS_D], t2[SS_D];
 #pragma unroll
-            for (int r = 0; r < SS_D; ++r) gam[r] = ss_add(gam[r], ss_scale(alpha[p][r], c[p + q]));
+        for (int r = 0; r < SS_D; ++r) t[r] = al[SS_ADJ_ORDER][r];
 #pragma unroll
-        for (int k = 0; k < SS_LIE; ++k) {
-            if (!(mask & (1u << k))) continue;
-            double e[4] = {0.0, 0.0, 0.0, 0.0};
-            e[k] = 1.0;
-            const ss_gen64 ek = ss_generator64(e, 1.0);
-            ss_cd t[SS_D];
-            ss_gen_vec(ek, 1.0, beta[q], t);
-            grad[k] += w * ss_re_dot(gam, t);
+        for (int p = 1; p <= SS_ADJ_ORDER; ++p) ss_axpy(lam, ss_inv_fact[p], al[p]);
+#pragma unroll
+        for (int p = SS_ADJ_ORDER + 1; p <= SS_ADJ_TAYLOR; ++p) {
+            ss_gen_vec(na, t, t2);
+#pragma unroll
+            for (int r = 0; r < SS_D; ++r) t[r] = t2[r];
+            ss_axpy(lam, ss_inv_fact[p], t);
         }
     }
 }
@@ -1241,14 +1300,57 @@ static __device__ __forceinline__ void ss_adjoint_exp(const double g[SS_FIELD_LE
     while (nu * ldexp(1.0, -split) > SS_ADJ_NU && split < SS_ADJ_MAX_SPLIT) ++split;
     if (nu * ldexp(1.0, -split) > SS_ADJ_NU) atomicOr(status, 1);
     const double scale = ldexp(1.0, -split);
-    const ss_gen64 a = ss_generator64(g, scale);
+    const ss_gen64 na = ss_generator64(g, -scale);  // -A
     const int pieces = 1 << split;
 #pragma unroll 1
-    for (int r = 0; r < pieces; ++r) {
-        ss_expm_neg_vec(a, psi);                         // state before this piece
-        ss_frechet_grad(a, lam, psi, mask, scale, grad);  // d/dg of the piece = scale * d/dg_s
-        ss_expm_neg_vec(a, lam);
+    for (int r = 0; r < pieces; ++r) ss_adjoint_piece(na, psi, lam, mask, scale, grad);  // d/dg = scale * d/dg_s
+}
+
+// Cubic Hermite interval i of a table: value = a_i h00 + a_{i+1} h01 + h L h10
+// + h R h11 in p = (t - t_i) / h, with the tangents L, R linear in a_{i-1} ..
+// a_{i+2} (:2108-2141): L = l_m1 a_{i-1} + l_0 a_i + l_1 a_{i+1},
+// R = r_0 a_i + r_1 a_{i+1} + r_2 a_{i+2}.
+struct ss_hermite_geom {
+    double h, l_m1, l_0, l_1, r_0, r_1, r_2;
+};
+
+static __device__ __forceinline__ ss_hermite_geom ss_hermite_geometry(const double* time, long long size, long long i)
+{
+    ss_hermite_geom g;
+    g.h = time[i + 1] - time[i];
+    const double dm0 = (time[i] != time[i + 1]) ? -1.0 / g.h : 0.0;  // d mid / d a_i
+    const double dm1 = -dm0;                                          // d mid / d a_{i+1}
+    g.l_m1 = g.l_0 = g.l_1 = g.r_0 = g.r_1 = g.r_2 = 0.0;
+    if (i == 0) {
+        g.l_0 = dm0;
+        g.l_1 = dm1;
+    } else if (time[i - 1] != time[i]) {
+        const double hl = time[i] - time[i - 1];
+        g.l_m1 = -0.5 / hl;
+        g.l_0 = 0.5 / hl + 0.5 * dm0;
+        g.l_1 = 0.5 * dm1;
     }
+    if (i + 2 == size) {
+        g.r_0 = dm0;
+        g.r_1 = dm1;
+    } else if (time[i + 1] != time[i + 2]) {
+        const double hr = time[i + 2] - time[i + 1];
+        g.r_0 = 0.5 * dm0;
+        g.r_1 = -0.5 / hr + 0.5 * dm1;
+        g.r_2 = 0.5 / hr;
+    }
+    return g;
+}
+
+// The four knot weights of interval i from the basis sums H00, H01, H10, H11
+// (each sum_s v_s times the basis polynomial at p_s, h included in H10, H11).
+static __device__ __forceinline__ void ss_hermite_weights(const ss_hermite_geom& g, double H00, double H01, double H10,
+                                                          double H11, double w[4])
+{
+    w[0] = H10 * g.l_m1;
+    w[1] = H00 + H10 * g.l_0 + H11 * g.r_0;
+    w[2] = H01 + H10 * g.l_1 + H11 * g.r_1;
+    w[3] = H11 * g.r_2;
 }
 
 // Interpolation weights of a table at t: value = sum_k w[k] amp[first + k].
@@ -1279,67 +1381,129 @@ static __device__ __forceinline__ void ss_interp_weights(const ss_table* tb, dou
         w[1] = p;
         return;
     }
-    // cubic Hermite: value = a_i h00 + a_{i+1} h01 + h L h10 + h R h11, with the
-    // tangents L, R linear in a_{i-1} .. a_{i+2} (:2108-2141)
-    const double h = time[i + 1] - time[i];
-    const double dm0 = (time[i] != time[i + 1]) ? -1.0 / h : 0.0;   // d mid / d a_i
-    const double dm1 = -dm0;                                        // d mid / d a_{i+1}
-    double l_m1 = 0, l_0 = 0, l_1 = 0, r_0 = 0, r_1 = 0, r_2 = 0;
-    if (i == 0) {
-        l_0 = dm0;
-        l_1 = dm1;
-    } else if (time[i - 1] != time[i]) {
-        const double hl = time[i] - time[i - 1];
-        l_m1 = -0.5 / hl;
-        l_0 = 0.5 / hl + 0.5 * dm0;
-        l_1 = 0.5 * dm1;
-    }
-    if (i + 2 == size) {
-        r_0 = dm0;
-        r_1 = dm1;
-    } else if (time[i + 1] != time[i + 2]) {
-        const double hr = time[i + 2] - time[i + 1];
-        r_0 = 0.5 * dm0;
-        r_1 = -0.5 / hr + 0.5 * dm1;
-        r_2 = 0.5 / hr;
-    }
+    const ss_hermite_geom g = ss_hermite_geometry(time, size, i);
     const double p2 = p * p, p3 = p2 * p;
-    const double h00 = 2 * p3 - 3 * p2 + 1, h01 = -2 * p3 + 3 * p2;
-    const double h10 = h * (p3 - 2 * p2 + p), h11 = h * (p3 - p2);
+    ss_hermite_weights(g, 2 * p3 - 3 * p2 + 1, -2 * p3 + 3 * p2, g.h * (p3 - 2 * p2 + p), g.h * (p3 - p2), w);
     *first = i - 1;
-    w[0] = h10 * l_m1;
-    w[1] = h00 + h10 * l_0 + h11 * r_0;
-    w[2] = h01 + h10 * l_1 + h11 * r_1;
-    w[3] = h11 * r_2;
 }
 
-// dL/d(lab-frame field) at one sample -> control gradients (LDS window or
-// atomics; in the deterministic mode a sample outside the window sets status
-// bit 2 instead, and the call fails).
+// Add vals[k] to the gradient of knot first + k of control c: the lane's LDS
+// window, else a float64 atomic (or, in the deterministic mode, status bit 2,
+// and the call fails).
+static __device__ __forceinline__ void ss_scatter_knots(const ss_adjoint_args& a, int c, long long first,
+                                                        const double vals[4], long long size, double* win,
+                                                        long long base_c, double* grad_row)
+{
+    const ss_control ct = a.controls[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long idx = first + k;
+        if (vals[k] == 0.0 || idx < 0 || idx >= size) continue;
+        const long long slot = idx - base_c;
+        if (slot >= 0 && slot < SS_ADJ_WINDOW)
+            win[(c * SS_ADJ_SLOTS + slot) * SS_ADJ_BLOCK] += vals[k];
+        else if (a.partial)
+            atomicOr(a.status, 2);
+        else if (ct.grad_offset + idx < a.grad_stride)
+            atomicAdd(grad_row + ct.grad_offset + idx, vals[k]);
+    }
+}
+
+// Gradient sums of the first SS_ADJ_CONTROLS controls while their samples stay
+// in one interval of a linear or cubic table: the knot weights are
+// polynomials in p of degree <= 3 whose coefficients depend on the interval
+// only, so sum_s v_s w_k(p_s) = sum_m C_km M_m with the moments
+// M_m = sum_s v_s p_s^m.  A lane adds four moments per sample (LDS slots
+// after its window) and turns them into knot gradients once per interval
+// (ss_flush_moments), instead of looking up weights and updating knots at
+// every sample.  A field source may define SS_ADJ_CONTROLS (1..4) to the
+// number of differentiated controls, so that only those interval indices are
+// kept in registers; further controls take the per-sample path.
+#ifndef SS_ADJ_CONTROLS
+#define SS_ADJ_CONTROLS SS_MAX_CONTROLS
+#endif
+
+static __device__ __forceinline__ double* ss_moment_slot(double* win, int c, int m)
+{
+    return win + (c * SS_ADJ_SLOTS + SS_ADJ_WINDOW + m) * SS_ADJ_BLOCK;
+}
+
+// Turn control c's moments of interval *idx into knot gradients and clear them.
+static __device__ __forceinline__ void ss_flush_moments(const ss_adjoint_args& a, const ss_ctx* ctx, int c,
+                                                        long long* idx, double* win, long long base_c,
+                                                        double* grad_row)
+{
+    if (*idx < 0) return;
+    const ss_table* tb = &ctx->tables[a.controls[c].table];
+    double mo[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        mo[m] = *ss_moment_slot(win, c, m);
+        *ss_moment_slot(win, c, m) = 0.0;
+    }
+    double vals[4] = {0.0, 0.0, 0.0, 0.0};
+    long long first = *idx;
+    if (tb->order == 1) {  // weights 1 - p, p
+        vals[0] = mo[0] - mo[1];
+        vals[1] = mo[1];
+    } else {  // cubic: h00 = 1 - 3p^2 + 2p^3, h01 = 3p^2 - 2p^3, h10 = h(p - 2p^2 + p^3), h11 = h(p^3 - p^2)
+        const ss_hermite_geom g = ss_hermite_geometry(tb->time, tb->size, *idx);
+        ss_hermite_weights(g, mo[0] - 3 * mo[2] + 2 * mo[3], 3 * mo[2] - 2 * mo[3],
+                           g.h * (mo[1] - 2 * mo[2] + mo[3]), g.h * (mo[3] - mo[2]), vals);
+        first = *idx - 1;
+    }
+    ss_scatter_knots(a, c, first, vals, tb->size, win, base_c, grad_row);
+    *idx = -1;
+}
+
+// dL/d(lab-frame field) at one sample -> control gradients: moment sums when
+// the control's table has its interval holding t cached (ss_ctx::line, set by
+// the field's own sample at t or a neighbouring one), else weights looked up
+// at t and added to the knots.
 static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& a, const ss_ctx* ctx, double t,
                                                          const double dfield[4], double* win,
-                                                         const long long base[SS_MAX_CONTROLS], double* grad_row)
+                                                         const long long base[SS_MAX_CONTROLS], double* grad_row,
+                                                         long long mo_idx[SS_ADJ_CONTROLS])
 {
-    for (int c = 0; c < a.n_controls; ++c) {
+#pragma unroll
+    for (int c = 0; c < SS_MAX_CONTROLS; ++c) {
+        if (c >= a.n_controls) break;
         const ss_control ct = a.controls[c];
         const double v = ct.scale * dfield[ct.axis];
+        if (c < SS_ADJ_CONTROLS && ct.table < SS_INTERP_LINES) {
+            // the cached interval of table ct.table (a uniform index: the
+            // copy is a uniform branch per line, no dynamic register index)
+            double t0 = 0.0, inv_h = __builtin_nan("");
+            long long idx = -1;
+#pragma unroll
+            for (int k = 0; k < SS_INTERP_LINES; ++k)
+                if (ct.table == k) {
+                    t0 = ctx->line[k].t0;
+                    inv_h = ctx->line[k].inv_h;
+                    idx = ctx->line[k].idx;
+                }
+            const double p = (t - t0) * inv_h;
+            if (p >= 0.0 && p <= 1.0) {
+                long long& mi = mo_idx[c < SS_ADJ_CONTROLS ? c : 0];
+                if (idx != mi) {
+                    ss_flush_moments(a, ctx, c, &mi, win, base[c], grad_row);
+                    mi = idx;
+                }
+                const double vp = v * p, vp2 = vp * p;
+                *ss_moment_slot(win, c, 0) += v;
+                *ss_moment_slot(win, c, 1) += vp;
+                *ss_moment_slot(win, c, 2) += vp2;
+                *ss_moment_slot(win, c, 3) = fma(vp2, p, *ss_moment_slot(win, c, 3));
+                continue;
+            }
+        }
         if (v == 0.0) continue;
         long long first;
         double w[4];
         ss_interp_weights(&ctx->tables[ct.table], t, &first, w);
-        const long long size = ctx->tables[ct.table].size;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const long long idx = first + k;
-            if (w[k] == 0.0 || idx < 0 || idx >= size) continue;
-            const long long slot = idx - base[c];
-            if (slot >= 0 && slot < SS_ADJ_WINDOW)
-                win[(c * SS_ADJ_WINDOW + slot) * SS_ADJ_BLOCK] += v * w[k];
-            else if (a.partial)
-                atomicOr(a.status, 2);
-            else if (ct.grad_offset + idx < a.grad_stride)
-                atomicAdd(grad_row + ct.grad_offset + idx, v * w[k]);
-        }
+        for (int k = 0; k < 4; ++k) w[k] *= v;
+        ss_scatter_knots(a, c, first, w, ctx->tables[ct.table].size, win, base[c], grad_row);
     }
 }
 
@@ -1365,16 +1529,17 @@ static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], ss_
 #endif
 }
 
-extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kernel(ss_adjoint_args a)
+extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) __attribute__((amdgpu_waves_per_eu(SS_ADJ_WAVES_PER_EU)))
+ss_batch_adjoint_kernel(ss_adjoint_args a)
 {
-    // n_controls * SS_ADJ_WINDOW slots per lane, sized at launch; slot k of a
+    // n_controls * SS_ADJ_SLOTS slots per lane, sized at launch; slot k of a
     // lane's window is ss_adj_windows[k * SS_ADJ_BLOCK + lane], so a wave's
     // accesses to one slot are consecutive doubles, free of LDS bank conflicts
     extern __shared__ double ss_adj_windows[];
     const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double* win = ss_adj_windows + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < a.n_controls * SS_ADJ_WINDOW; ++k) win[k * SS_ADJ_BLOCK] = 0.0;
+    for (int k = 0; k < a.n_controls * SS_ADJ_SLOTS; ++k) win[k * SS_ADJ_BLOCK] = 0.0;
     if (row >= a.total) return;
     long long lo = 0, hi = a.n_sims;
     while (hi - lo > 1) {
@@ -1414,6 +1579,10 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
 #else
     (void)z_control;
 #endif
+
+    long long mo[SS_ADJ_CONTROLS];  // interval of each control's moment sums, -1: none
+#pragma unroll
+    for (int c = 0; c < SS_ADJ_CONTROLS; ++c) mo[c] = -1;
 
     // state and costate at the interval's end, in the rotating frame
     ss_cd u[SS_D][SS_D], psi0[SS_D], psi[SS_D], lam[SS_D];
@@ -1493,7 +1662,7 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
         if (z_control) dz_mid += tra * (dfa[0] * fa[1] - dfa[1] * fa[0]) - dfa[2];
 #endif
         ss_unframe(dfa, wa_w, lab);
-        ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
+        ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row, mo);
 #else
         double fb[SS_FIELD_LEN], g2[SS_FIELD_LEN], d2[SS_LIE], dfb[SS_LIE];
         const double trb = wb.t_rel;
@@ -1535,24 +1704,27 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
                       trb * (dfb[0] * fb[1] - dfb[1] * fb[0]) - dfb[2];
 #endif
         ss_unframe(dfa, wa_w, lab);
-        ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row);
+        ss_scatter_sample(a, &ctx, tra + tc, lab, win, base, grad_row, mo);
         ss_unframe(dfb, wb_w, lab);
-        ss_scatter_sample(a, &ctx, trb + tc, lab, win, base, grad_row);
+        ss_scatter_sample(a, &ctx, trb + tc, lab, win, base, grad_row, mo);
 #endif
     }
 #if SS_ROT
     if (z_control) {
         const double dmid[4] = {0.0, 0.0, dz_mid, 0.0};
-        ss_scatter_sample(a, &ctx, tc + dt_out / 2, dmid, win, base, grad_row);
+        ss_scatter_sample(a, &ctx, tc + dt_out / 2, dmid, win, base, grad_row, mo);
     }
 #endif
+#pragma unroll
+    for (int c = 0; c < SS_ADJ_CONTROLS; ++c)
+        if (c < a.n_controls) ss_flush_moments(a, &ctx, c, &mo[c], win, base[c], grad_row);
     if (a.partial) {
         // deterministic mode: the window goes to this row's record; a second
         // pass sums the records of each knot in row order
         for (int c = 0; c < a.n_controls; ++c) {
             double* rec = a.partial + (row * a.n_controls + c) * SS_ADJ_WINDOW;
 #pragma unroll
-            for (int s = 0; s < SS_ADJ_WINDOW; ++s) rec[s] = win[(c * SS_ADJ_WINDOW + s) * SS_ADJ_BLOCK];
+            for (int s = 0; s < SS_ADJ_WINDOW; ++s) rec[s] = win[(c * SS_ADJ_SLOTS + s) * SS_ADJ_BLOCK];
             a.window_base[row * a.n_controls + c] = base[c];
         }
         return;
@@ -1562,7 +1734,7 @@ extern "C" __global__ void __launch_bounds__(SS_ADJ_BLOCK) ss_batch_adjoint_kern
 #pragma unroll
         for (int s = 0; s < SS_ADJ_WINDOW; ++s) {
             const long long idx = base[c] + s;
-            const double v = win[(c * SS_ADJ_WINDOW + s) * SS_ADJ_BLOCK];
+            const double v = win[(c * SS_ADJ_SLOTS + s) * SS_ADJ_BLOCK];
             if (v != 0.0 && idx >= 0 && idx < size && a.controls[c].grad_offset + idx < a.grad_stride)
                 atomicAdd(grad_row + a.controls[c].grad_offset + idx, v);
         }

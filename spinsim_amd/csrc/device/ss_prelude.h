@@ -41,9 +41,11 @@ struct ss_table {
 };
 
 // The interval a table was last sampled in, as the cubic
-// c0 + c1 p + c2 p^2 + c3 p^3 of p = (t - t0) inv_h; inv_h = NaN when empty.
+// c0 + c1 p + c2 p^2 + c3 p^3 of p = (t - t0) inv_h; inv_h = NaN when empty;
+// idx is the interval's index (knots idx, idx + 1).
 struct ss_interp_line {
     double t0, inv_h, c0, c1, c2, c3;
+    long long idx;
 };
 
 #ifndef SS_INTERP_LINES
@@ -60,7 +62,10 @@ struct ss_ctx {
     __device__ ss_ctx() : tables(nullptr), sim(0)
     {
 #pragma unroll
-        for (int k = 0; k < SS_INTERP_LINES; ++k) line[k].inv_h = __builtin_nan("");
+        for (int k = 0; k < SS_INTERP_LINES; ++k) {
+            line[k].inv_h = __builtin_nan("");
+            line[k].idx = -1;
+        }
     }
 };
 
@@ -392,6 +397,7 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
                 ss_interp_line& ln = ctx->line[k];
                 ln.t0 = time[i];
                 ln.inv_h = 1.0 / h;
+                ln.idx = i;
                 ln.c0 = amp[i];
                 ln.c1 = amp[i + 1] - amp[i];
                 ln.c2 = 0.0;
@@ -409,6 +415,7 @@ __device__ static inline double ss_interp_sample(const ss_ctx* ctx, int k, doubl
             ss_interp_line& ln = ctx->line[k];
             ln.t0 = time[i];
             ln.inv_h = 1.0 / h;
+            ln.idx = i;
             ln.c0 = amp[i];
             ln.c1 = left;
             ln.c2 = 3.0 * (amp[i + 1] - amp[i]) - 2.0 * left - right;

@@ -1728,7 +1728,7 @@ int ss_batch_adjoint(const ss_module* m, const double* sweep, int64_t sweep_stri
     if (blocks > 2147483647LL) return fail("ss_batch_adjoint: grid too large");
     // the per-lane gradient windows: only as much LDS as the controls need,
     // so fewer controls leave room for more workgroups per CU
-    const unsigned lds = unsigned(n_controls) * SS_ADJ_WINDOW * SS_ADJ_BLOCK * sizeof(double);
+    const unsigned lds = unsigned(n_controls) * SS_ADJ_SLOTS * SS_ADJ_BLOCK * sizeof(double);
     void* params[] = {&a};
     SS_HIP(hipModuleLaunchKernel(m->adjoint, unsigned(blocks), 1, 1, SS_ADJ_BLOCK, 1, 1, lds, s, params,
                                  nullptr));
