@@ -1401,7 +1401,8 @@ static __device__ __forceinline__ void ss_scatter_knots(const ss_adjoint_args& a
         if (vals[k] == 0.0 || idx < 0 || idx >= size) continue;
         const long long slot = idx - base_c;
         if (slot >= 0 && slot < SS_ADJ_WINDOW)
-            win[(c * SS_ADJ_SLOTS + slot) * SS_ADJ_BLOCK] += vals[k];
+            __hip_atomic_fetch_add(win + (c * SS_ADJ_SLOTS + slot) * SS_ADJ_BLOCK, vals[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         else if (a.partial)
             atomicOr(a.status, 2);
         else if (ct.grad_offset + idx < a.grad_stride)
@@ -1489,11 +1490,14 @@ static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& 
                     ss_flush_moments(a, ctx, c, &mi, win, base[c], grad_row);
                     mi = idx;
                 }
+                // LDS adds without return (ds_add_f64): the lane's own slots,
+                // so no read sits on the sample's dependency chain
                 const double vp = v * p, vp2 = vp * p;
-                *ss_moment_slot(win, c, 0) += v;
-                *ss_moment_slot(win, c, 1) += vp;
-                *ss_moment_slot(win, c, 2) += vp2;
-                *ss_moment_slot(win, c, 3) = fma(vp2, p, *ss_moment_slot(win, c, 3));
+                __hip_atomic_fetch_add(ss_moment_slot(win, c, 0), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(ss_moment_slot(win, c, 1), vp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(ss_moment_slot(win, c, 2), vp2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(ss_moment_slot(win, c, 3), vp2 * p, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                 continue;
             }
         }
