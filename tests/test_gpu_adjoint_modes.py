@@ -73,8 +73,16 @@ def test_deterministic_refuses_windows_it_cannot_hold():
     psi0 = ss.SpinQuantumNumber.ONE.plus_z
     with pytest.raises(RuntimeError, match="8-knot gradient window"):
         _grad(cs, amp, t1, psi0, True)
-    g, _ = _grad(cs, amp, t1, psi0, False)  # the atomic mode takes any knot spacing
+    g, psi = _grad(cs, amp, t1, psi0, False)  # the atomic mode takes any knot spacing
     assert np.isfinite(g).all() and np.abs(g).max() > 0
+    # 20 intervals per coarse step: the moment sums flush at every knot
+    for s in range(2):
+        psi_ref, g_ref = AR.final_state_and_gradient(_static_z_one, "one", [(0, tk, amp[s, 0], 3, True, 1.0)],
+                                                     0.0, t1, 20e-9, 1e-6, psi0, _psi_grad)
+        assert np.abs(psi[s] - psi_ref).max() < 1e-9
+        err = np.abs(g[s, 0] - g_ref[0]).max() / np.abs(g_ref[0]).max()
+        print(f"  sim {s}: dense knots, relative error vs CPU reference {err:.3e}")
+        assert err < 1e-8
 
 
 @pytest.mark.parametrize("deterministic", [False, True])
