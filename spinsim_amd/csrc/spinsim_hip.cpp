@@ -8,6 +8,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -560,24 +561,35 @@ __device__ __forceinline__ void shfl_aff(const ss_cd m[D][D], const ss_cd c[D], 
 
 struct ScanShared {
     long long next_sim;   // first simulation (scan order) not yet marked
+    long long next_start; // its start row (LLONG_MAX past the last): tiles before it start none
     int count;
     int pmin;
 };
 
+template <bool REV>
+__device__ __forceinline__ long long scan_next_start(const ScanArgs& a, long long i)
+{
+    return i < a.n_sims ? scan_start<REV>(a, i) : LLONG_MAX;
+}
+
 // start_sim[i] = the simulation starting at tile row r0 + i, or -1, for the
 // rows [r0, r1); sh.next_sim moves past every simulation starting before r1.
-// Returns whether any row of the tile starts a simulation.
+// Returns whether any row of the tile starts a simulation.  A tile before
+// sh.next_start starts none and returns at once (no loads, no barriers; its
+// start_sim is then stale, and only the affine path reads it).
 template <bool REV, int R>
 __device__ bool scan_mark_starts(const ScanArgs& a, long long r0, long long r1, int* start_sim, ScanShared& sh)
 {
     const int t = threadIdx.x;
+    if (a.offsets ? sh.next_start >= r1 : r0 != 0) return false;
+    __syncthreads();  // the tiles before have finished reading start_sim
 #pragma unroll
     for (int k = 0; k < R; ++k) start_sim[k * kScanThreads + t] = -1;
     __syncthreads();
-    if (!a.offsets) {
-        if (r0 == 0 && t == 0) start_sim[0] = 0;
+    if (!a.offsets) {  // one simulation, starting at row 0
+        if (t == 0) start_sim[0] = 0;
         __syncthreads();
-        return r0 == 0;
+        return true;
     }
     int any = 0;
 #pragma unroll 1
@@ -600,6 +612,7 @@ __device__ bool scan_mark_starts(const ScanArgs& a, long long r0, long long r1, 
         __syncthreads();
         if (cnt < kScanThreads) break;
     }
+    if (t == 0) sh.next_start = scan_next_start<REV>(a, sh.next_sim);
     return __syncthreads_or(any);
 }
 
@@ -724,10 +737,14 @@ __global__ void __launch_bounds__(kScanThreads) batch_scan_reduce(ScanArgs a)
     const long long c0 = j * a.chunk_rows;
     const long long c1 = c0 + a.chunk_rows < a.total ? c0 + a.chunk_rows : a.total;
     if (t == 0) {
-        if (a.offsets) sh.next_sim = scan_first_sim<REV>(a, c0);
+        if (a.offsets) {
+            sh.next_sim = scan_first_sim<REV>(a, c0);
+            sh.next_start = scan_next_start<REV>(a, sh.next_sim);
+        }
         ss_set_identity<D>(gm);
         set_zero_vec<D>(gc);
     }
+    __syncthreads();
     bool constant = false;
 #pragma unroll 1
     for (long long r0 = c0; r0 < c1; r0 += kScanThreads * R) {
@@ -901,7 +918,11 @@ __global__ void __launch_bounds__(kScanThreads) batch_scan_apply(ScanArgs a)
     const long long j = blockIdx.x;
     const long long c0 = j * a.chunk_rows;
     const long long c1 = c0 + a.chunk_rows < a.total ? c0 + a.chunk_rows : a.total;
-    if (t == 0 && a.offsets) sh.next_sim = scan_first_sim<REV>(a, c0);
+    if (t == 0 && a.offsets) {
+        sh.next_sim = scan_first_sim<REV>(a, c0);
+        sh.next_start = scan_next_start<REV>(a, sh.next_sim);
+    }
+    __syncthreads();
     ss_cd vin[D];
     set_zero_vec<D>(vin);
     if (j > 0) scan_incoming<D>(a, j, vin, sh, s_tm, s_v);
