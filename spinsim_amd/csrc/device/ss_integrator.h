@@ -1120,10 +1120,7 @@ extern "C" __global__ void SS_KERNEL_ATTR ss_batch_time_evolution_kernel(ss_batc
 // f the transformed sample, psi and lam at the interval's end; the sum is
 // scattered onto the z controls at the midpoint.
 // ===========================================================================
-#ifndef SS_ADJ_ORDER
-#define SS_ADJ_ORDER 5
-#endif
-//       // Frechet series: terms p + q <= 5
+#define SS_ADJ_ORDER 5       // Frechet series: terms p + q <= 5
 #define SS_ADJ_TAYLOR 7      // Taylor order of the exponentials walking psi and lambda back
 #define SS_ADJ_NU 0.035      // sub-step until the generator norm bound is below this
 #define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 35.8)
