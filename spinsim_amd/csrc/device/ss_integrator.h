@@ -452,6 +452,30 @@ static __device__ __forceinline__ void ss_append(const double g[SS_FIELD_LEN], s
     ss_left_multiply<SS_D>(e, u);
 }
 
+#if SS_DIM == 2
+// Spin-half step operators are special unitary, [[p, -conj q], [q, conj p]],
+// and so are their products, so a chunk's operator is accumulated as its first
+// column (p, q): e u = (e00 p - conj(e10) q, e10 p + conj(e00) q), four complex
+// products where the full 2x2 product takes eight.
+static __device__ __forceinline__ void ss_append_su2(const double g[SS_FIELD_LEN], ss_cd& p, ss_cd& q)
+{
+    ss_cd e[2][2];
+    ss_expo(g, e);
+    const ss_cd np = ss_mac(ss_mul(e[0][0], p), ss_c(-e[1][0].x, e[1][0].y), q);
+    const ss_cd nq = ss_mac(ss_mul(e[1][0], p), ss_conj(e[0][0]), q);
+    p = np;
+    q = nq;
+}
+
+static __device__ __forceinline__ void ss_su2_expand(ss_cd p, ss_cd q, ss_cd u[2][2])
+{
+    u[0][0] = p;
+    u[1][0] = q;
+    u[0][1] = ss_c(-q.x, q.y);
+    u[1][1] = ss_conj(p);
+}
+#endif
+
 #if SS_PREC != 0
 // ---- float32-bulk step operators -------------------------------------------
 // A step operator is E = exp(A), A = -i G(g).  Its first-order part A is kept
@@ -683,6 +707,46 @@ static __device__ __forceinline__ void ss_expo_residual(const double* g, ss_cf b
 
 // P <- exp(A(g)) P for P = I + A(gsum) + b:  gsum += g,
 // b <- b + B + (A(g) + B)(A(gsum_old) + b).
+#if SS_DIM == 2
+// Spin half: b, the step's residual bb, X = A(g) + bb and P = A(gsum) + b all
+// have the form [[p, -conj q], [q, conj p]] (A's first column is (i d0, l)),
+// so only first columns are kept (b[0][0], b[1][0]; the rest of b stays zero
+// and unused) and X P takes four complex products.
+static __device__ __forceinline__ ss_cf ss_fmul(ss_cf a, ss_cf b)
+{
+    return ss_f(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+static __device__ __forceinline__ void ss_append_mixed(const double g[SS_FIELD_LEN], double gsum[SS_LIE],
+                                                       ss_cf b[SS_D][SS_D])
+{
+    ss_cf bb[SS_D][SS_D];
+    ss_expo_residual(g, bb);
+    const ss_gen a = ss_generator(g, 1.0), s = ss_generator(gsum, 1.0);
+    const ss_cf xp = ss_f(bb[0][0].x, a.d[0] + bb[0][0].y), xq = ss_f(a.l.x + bb[1][0].x, a.l.y + bb[1][0].y);
+    const ss_cf pp = ss_f(b[0][0].x, s.d[0] + b[0][0].y), pq = ss_f(s.l.x + b[1][0].x, s.l.y + b[1][0].y);
+    // (X P) first column: (xp pp - conj(xq) pq, xq pp + conj(xp) pq)
+    const ss_cf wp = ss_fmac(ss_fmul(xp, pp), ss_f(-xq.x, xq.y), pq);
+    const ss_cf wq = ss_fmac(ss_fmul(xq, pp), ss_f(xp.x, -xp.y), pq);
+    b[0][0] = ss_f(b[0][0].x + bb[0][0].x + wp.x, b[0][0].y + bb[0][0].y + wp.y);
+    b[1][0] = ss_f(b[1][0].x + bb[1][0].x + wq.x, b[1][0].y + bb[1][0].y + wq.y);
+#pragma unroll
+    for (int k = 0; k < SS_LIE; ++k) gsum[k] += g[k];
+}
+
+// U = I + A(gsum) + b in float64, from the first columns.
+static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], const ss_cf b[SS_D][SS_D],
+                                                     ss_cd u[SS_D][SS_D])
+{
+    const double hx = 0.5 * gsum[0], hy = 0.5 * gsum[1], hz = 0.5 * gsum[2];
+    const ss_cd p = ss_c(1.0 + (double)b[0][0].x, -hz + (double)b[0][0].y);
+    const ss_cd q = ss_c(hy + (double)b[1][0].x, -hx + (double)b[1][0].y);
+    u[0][0] = p;
+    u[1][0] = q;
+    u[0][1] = ss_c(-q.x, q.y);
+    u[1][1] = ss_conj(p);
+}
+#else
 static __device__ __forceinline__ void ss_append_mixed(const double g[SS_FIELD_LEN], double gsum[SS_LIE],
                                                        ss_cf b[SS_D][SS_D])
 {
@@ -712,15 +776,9 @@ static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], 
                                                      ss_cd u[SS_D][SS_D])
 {
     double gg[4] = {gsum[0], gsum[1], gsum[2], SS_LIE > 3 ? gsum[3] : 0.0};
-#if SS_DIM == 2
-    const double hx = 0.5 * gg[0], hy = 0.5 * gg[1], hz = 0.5 * gg[2];
-    const double d[2] = {-hz, hz};
-    const ss_cd up = ss_c(-hy, -hx), lo = ss_c(hy, -hx);
-#else
     const double rx = gg[0] * SS_INV_SQRT2, ry = gg[1] * SS_INV_SQRT2, z = gg[2], q3 = gg[3] * (1.0 / 3.0);
     const double d[3] = {-(z + q3), 2.0 * q3, z - q3};
     const ss_cd up = ss_c(-ry, -rx), lo = ss_c(ry, -rx);
-#endif
 #pragma unroll
     for (int r = 0; r < SS_D; ++r)
 #pragma unroll
@@ -729,6 +787,7 @@ static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], 
             u[r][c] = ss_c(a.x + (double)b[r][c].x, a.y + (double)b[r][c].y);
         }
 }
+#endif  // SS_DIM
 #endif
 
 // Winding exp(i rot t_rel) of a sample, advanced from step to step by
@@ -775,6 +834,9 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #pragma unroll
         for (int c = 0; c < SS_D; ++c) bres[r][c] = ss_f((ss_bulk)0, (ss_bulk)0);
 #define SS_APPEND(g) ss_append_mixed(g, gsum, bres)
+#elif SS_DIM == 2
+    ss_cd su_p = ss_c(1.0, 0.0), su_q = ss_c(0.0, 0.0);
+#define SS_APPEND(g) ss_append_su2(g, su_p, su_q)
 #else
 #define SS_APPEND(g) ss_append(g, u)
 #endif
@@ -833,6 +895,8 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
     }
 #if SS_PREC != 0
     ss_mixed_to_u(gsum, bres, u);
+#elif SS_DIM == 2
+    ss_su2_expand(su_p, su_q, u);
 #endif
 #undef SS_APPEND
 }
