@@ -101,11 +101,16 @@ enum { /* ExponentiationMethod, :95-129 */
 
 enum { /* layout of the batch's operator rows (no reference counterpart) */
     SS_U_FULL = 0,      /* (total, dim, dim) complex128: every entry */
-    SS_U_SU2_COLUMN = 1 /* spin half only: (total, 2) complex128, the first column
-                           (a, b) of each U = [[a, -conj b], [b, conj a]] (the
-                           step operators of spin half are special unitary);
-                           half the bytes, for batches whose operators feed the
-                           state scan and are not read as matrices */
+    SS_U_SU2_COLUMN = 1, /* spin half only: (total, 2) complex128, the first column
+                            (a, b) of each U = [[a, -conj b], [b, conj a]] (the
+                            step operators of spin half are special unitary);
+                            half the bytes, for batches whose operators feed the
+                            state scan and are not read as matrices */
+    SS_U_SU2_FULL = 2    /* spin half only: (total, 2, 2) like SS_U_FULL, every
+                            operator special unitary (as ss_batch_time_evolution
+                            writes them): the state scan reads first columns and
+                            composes maps in that form (four complex products
+                            where a 2x2 product takes eight) */
 };
 
 /* Message of the last failure on this thread ("" if none). */
@@ -169,9 +174,9 @@ int ss_time_evolution_plan(int64_t n_coarse, int n_fine, int64_t waves, int *n_p
  * of time_out / u_out and reads sweep row s (sweep + s*sweep_stride).  All
  * arrays are device arrays; offsets has n_sims+1 entries, offsets[n_sims] =
  * total.  counter: one device uint64 the library clears before the launch.
- * grid_blocks = 0 picks a persistent grid for the device.  u_layout: SS_U_FULL,
- * or SS_U_SU2_COLUMN for a spin-half module (u_out then holds total*2
- * complex128). */
+ * grid_blocks = 0 picks a persistent grid for the device.  u_layout: SS_U_FULL
+ * (or SS_U_SU2_FULL, the same store), or SS_U_SU2_COLUMN for a spin-half
+ * module (u_out then holds total*2 complex128). */
 int ss_batch_time_evolution(const ss_module *m, const double *sweep, int64_t sweep_stride,
                             int64_t n_sims, const int64_t *offsets, const double *t0,
                             const double *dt, const double *dt_out, const int32_t *n_fine,
@@ -202,7 +207,8 @@ int ss_state_scan(const double *psi0, const double *u, double *state, double *sp
  * device (n_sims, dim); spin: NULL or device (total, 3).  chunk_rows: rows per
  * workgroup chunk, 0 for the default or a positive multiple of 1024; the
  * workspace holds ss_batch_scan_workspace(total, dim, chunk_rows) bytes.
- * u_layout: how u is stored (SS_U_FULL; SS_U_SU2_COLUMN with dim 2). */
+ * u_layout: how u is stored (SS_U_FULL; SS_U_SU2_COLUMN or SS_U_SU2_FULL with
+ * dim 2, for special unitary operators only). */
 int ss_batch_state_scan(const double *psi0, const double *u, double *state, double *spin,
                         const int64_t *offsets, int64_t n_sims, int64_t total, int dim,
                         int64_t chunk_rows, int u_layout, void *workspace,

@@ -69,6 +69,7 @@ ABI_VERSION = 9
 # Operator row layouts of the batch entry points (SS_U_* of the C header).
 U_FULL = 0
 U_SU2_COLUMN = 1
+U_SU2_FULL = 2
 
 
 class NativeError(RuntimeError):

@@ -127,6 +127,10 @@ class BatchPlan:
             raise ValueError("compact operator storage is for spin half (dimension 2)")
         self.compact = bool(compact)
         self.u_layout = _native.U_SU2_COLUMN if compact else _native.U_FULL
+        # the state scan composes spin-half operators (special unitary) as
+        # first columns, from either store
+        self.scan_layout = (_native.U_SU2_COLUMN if compact else
+                            _native.U_SU2_FULL if d == 2 else _native.U_FULL)
         sweep = np.asarray(sweep_parameters, np.float64)
         if sweep.ndim == 1:
             sweep = sweep[None, :]
@@ -220,7 +224,7 @@ class BatchPlan:
         _native.check(_native.lib().ss_batch_state_scan(
             self.psi0.data_ptr(), self.u.data_ptr(), self.state.data_ptr(),
             None if spin is None else spin.data_ptr(), self.offsets.data_ptr(), self.n_sims,
-            self.total, self.sim.dimension, self.scan_chunk_rows, self.u_layout,
+            self.total, self.sim.dimension, self.scan_chunk_rows, self.scan_layout,
             self.scan_ws.data_ptr(), self.scan_ws.numel(), stream),
             "ss_batch_state_scan")
 

@@ -436,6 +436,18 @@ __device__ __forceinline__ void load_vec_c(const double* p, ss_cd v[D])
 
 // Matrix leading into scan-order row r (r >= 1): U[r - 1] forward,
 // U[total - r]^+ backward (costate[i] = U[i+1]^+ costate[i+1]).
+// Q (spin half, special unitary operators): every map matrix has the form
+// [[p, -conj q], [q, conj p]] (operators, their products, identity and zero),
+// kept expanded in registers but computed and shuffled as its first column.
+template <int D>
+__device__ __forceinline__ void su2_expand(ss_cd m[D][D], ss_cd p, ss_cd q)
+{
+    m[0][0] = p;
+    m[1][0] = q;
+    m[0][1] = ss_c(-q.x, q.y);
+    m[1][1] = ss_conj(p);
+}
+
 // Spin-half operator k from its first column (a, b): U = [[a, -conj b], [b, conj a]].
 template <int D>
 __device__ __forceinline__ void load_su2_column(const double* u, long long k, ss_cd m[D][D])
@@ -453,6 +465,10 @@ __device__ __forceinline__ void scan_row_mat(const ScanArgs& a, long long r, ss_
 {
     if (D == 2 && !REV && a.u_layout == SS_U_SU2_COLUMN) {
         load_su2_column<D>(a.u, r - 1, m);
+    } else if (D == 2 && !REV && a.u_layout == SS_U_SU2_FULL) {
+        const double2* p = reinterpret_cast<const double2*>(a.u) + (r - 1) * 4;
+        const double2 c0 = p[0], c1 = p[2];
+        su2_expand<D>(m, ss_c(c0.x, c0.y), ss_c(c1.x, c1.y));
     } else if (!REV) {
         load_mat<D>(a.u, r - 1, m);
     } else {
@@ -485,9 +501,21 @@ __device__ __forceinline__ void set_zero_vec(ss_cd v[D])
 
 // Affine map v -> m v + c (c = 0 throughout when AFF is false).
 // (m, c) <- (lm, lc) o (m, c), in place a column at a time.
-template <int D, bool AFF>
+template <int D, bool AFF, bool Q = false>
 __device__ __forceinline__ void aff_after(const ss_cd lm[D][D], const ss_cd lc[D], ss_cd m[D][D], ss_cd c[D])
 {
+    if constexpr (Q && D == 2) {
+        const ss_cd p = ss_mac(ss_mul(lm[0][0], m[0][0]), lm[0][1], m[1][0]);
+        const ss_cd q = ss_mac(ss_mul(lm[1][0], m[0][0]), lm[1][1], m[1][0]);
+        if (AFF) {
+            const ss_cd c0 = ss_mac(ss_mac(lc[0], lm[0][0], c[0]), lm[0][1], c[1]);
+            const ss_cd c1 = ss_mac(ss_mac(lc[1], lm[1][0], c[0]), lm[1][1], c[1]);
+            c[0] = c0;
+            c[1] = c1;
+        }
+        su2_expand<D>(m, p, q);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         ss_cd col[D];
@@ -516,9 +544,19 @@ __device__ __forceinline__ void aff_after(const ss_cd lm[D][D], const ss_cd lc[D
 }
 
 // (m, c) <- (m, c) o (pm, pc), in place a row at a time.
-template <int D, bool AFF>
+template <int D, bool AFF, bool Q = false>
 __device__ __forceinline__ void aff_before(ss_cd m[D][D], ss_cd c[D], const ss_cd pm[D][D], const ss_cd pc[D])
 {
+    if constexpr (Q && D == 2) {
+        const ss_cd p = ss_mac(ss_mul(m[0][0], pm[0][0]), m[0][1], pm[1][0]);
+        const ss_cd q = ss_mac(ss_mul(m[1][0], pm[0][0]), m[1][1], pm[1][0]);
+        if (AFF) {
+            c[0] = ss_mac(ss_mac(c[0], m[0][0], pc[0]), m[0][1], pc[1]);
+            c[1] = ss_mac(ss_mac(c[1], m[1][0], pc[0]), m[1][1], pc[1]);
+        }
+        su2_expand<D>(m, p, q);
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         ss_cd row[D];
@@ -540,13 +578,13 @@ __device__ __forceinline__ void aff_before(ss_cd m[D][D], ss_cd c[D], const ss_c
     }
 }
 
-template <int D, bool AFF, bool UP>
+template <int D, bool AFF, bool UP, bool Q = false>
 __device__ __forceinline__ void shfl_aff(const ss_cd m[D][D], const ss_cd c[D], ss_cd om[D][D], ss_cd oc[D], int s)
 {
 #pragma unroll
     for (int r = 0; r < D; ++r) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
+        for (int k = 0; k < ((Q && D == 2) ? 1 : D); ++k) {
             om[r][k].x = UP ? __shfl_up(m[r][k].x, s, 64) : __shfl_down(m[r][k].x, s, 64);
             om[r][k].y = UP ? __shfl_up(m[r][k].y, s, 64) : __shfl_down(m[r][k].y, s, 64);
         }
@@ -557,6 +595,7 @@ __device__ __forceinline__ void shfl_aff(const ss_cd m[D][D], const ss_cd c[D], 
             oc[r] = ss_c(0.0, 0.0);
         }
     }
+    if constexpr (Q && D == 2) su2_expand<D>(om, om[0][0], om[1][0]);
 }
 
 struct ScanShared {
@@ -659,7 +698,7 @@ __device__ __forceinline__ void scan_row_map_loaded(const ScanArgs& a, long long
 // cache lines it reads are then used back to back instead of being left to
 // other waves' traffic between rows (which refetched 44% more than the
 // operators in the batch reduce pass).
-template <int D, bool REV, int R, bool AFF>
+template <int D, bool REV, int R, bool AFF, bool Q = false>
 __device__ __forceinline__ void scan_thread_map(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
                                                 ss_cd xm[D][D], ss_cd xc[D])
 {
@@ -679,7 +718,7 @@ __device__ __forceinline__ void scan_thread_map(const ScanArgs& a, long long r0,
         for (int k = 1; k < R; ++k) {
             ss_cd m[D][D], c[D];
             scan_row_map_loaded<D, AFF>(a, base + k, r1, start_sim[t * R + k], mats[k], m, c);
-            aff_after<D, AFF>(m, c, xm, xc);
+            aff_after<D, AFF, Q>(m, c, xm, xc);
         }
     } else {
         scan_row_map<D, REV, AFF>(a, base, r1, start_sim[t * R], xm, xc);
@@ -687,25 +726,25 @@ __device__ __forceinline__ void scan_thread_map(const ScanArgs& a, long long r0,
         for (int k = 1; k < R; ++k) {
             ss_cd m[D][D], c[D];
             scan_row_map<D, REV, AFF>(a, base + k, r1, start_sim[t * R + k], m, c);
-            aff_after<D, AFF>(m, c, xm, xc);
+            aff_after<D, AFF, Q>(m, c, xm, xc);
         }
     }
 }
 
 // Pass 1 on one tile: (gm, gc) <- (tile map) o (gm, gc), kept in LDS.
-template <int D, bool REV, int R, bool AFF>
+template <int D, bool REV, int R, bool AFF, bool Q>
 __device__ void scan_tile_reduce(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
                                  ss_cd (&s_tm)[4][D][D], ss_cd (&s_tc)[4][D], ss_cd (&gm)[D][D], ss_cd (&gc)[D])
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     ss_cd xm[D][D], xc[D];
-    scan_thread_map<D, REV, R, AFF>(a, r0, r1, start_sim, xm, xc);
+    scan_thread_map<D, REV, R, AFF, Q>(a, r0, r1, start_sim, xm, xc);
     // x_l <- x_{l+s} o x_l: lane 0 ends with the wave's map (later rows on the left)
 #pragma unroll 1
     for (int s = 1; s < 64; s <<= 1) {
         ss_cd om[D][D], oc[D];
-        shfl_aff<D, AFF, false>(xm, xc, om, oc, s);
-        if ((lane & (2 * s - 1)) == 0) aff_after<D, AFF>(om, oc, xm, xc);
+        shfl_aff<D, AFF, false, Q>(xm, xc, om, oc, s);
+        if ((lane & (2 * s - 1)) == 0) aff_after<D, AFF, Q>(om, oc, xm, xc);
     }
     if (lane == 0) {
         ss_copy<D>(xm, s_tm[w]);
@@ -718,12 +757,12 @@ __device__ void scan_tile_reduce(const ScanArgs& a, long long r0, long long r1, 
         ss_cd zc[D];
         set_zero_vec<D>(zc);
 #pragma unroll 1
-        for (int ww = 0; ww < 4; ++ww) aff_after<D, true>(s_tm[ww], AFF ? s_tc[ww] : zc, gm, gc);
+        for (int ww = 0; ww < 4; ++ww) aff_after<D, true, Q>(s_tm[ww], AFF ? s_tc[ww] : zc, gm, gc);
     }
     __syncthreads();
 }
 
-template <int D, bool REV, int R>
+template <int D, bool REV, int R, bool Q = false>
 __global__ void __launch_bounds__(kScanThreads) batch_scan_reduce(ScanArgs a)
 {
     __shared__ ScanShared sh;
@@ -750,10 +789,10 @@ __global__ void __launch_bounds__(kScanThreads) batch_scan_reduce(ScanArgs a)
     for (long long r0 = c0; r0 < c1; r0 += kScanThreads * R) {
         const long long r1 = r0 + kScanThreads * R < c1 ? r0 + kScanThreads * R : c1;
         if (scan_mark_starts<REV, R>(a, r0, r1, start_sim, sh)) {
-            scan_tile_reduce<D, REV, R, true>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
+            scan_tile_reduce<D, REV, R, true, Q>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
             constant = true;
         } else {
-            scan_tile_reduce<D, REV, R, false>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
+            scan_tile_reduce<D, REV, R, false, Q>(a, r0, r1, start_sim, s_tm, s_tc, gm, gc);
         }
     }
     if (t == 0) {
@@ -834,19 +873,19 @@ __device__ void scan_incoming(const ScanArgs& a, long long j, ss_cd vin[D], Scan
 
 // Pass 2 on one tile: states of rows [r0, r1) from the incoming state vin
 // (updated to the tile's last state).
-template <int D, bool REV, int R, bool AFF>
+template <int D, bool REV, int R, bool AFF, bool Q>
 __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, const int* start_sim,
                                 ss_cd (&s_tm)[4][D][D], ss_cd (&s_tc)[4][D], ss_cd (&s_v)[D], ss_cd vin[D])
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     ss_cd qm[D][D], qc[D];
-    scan_thread_map<D, REV, R, AFF>(a, r0, r1, start_sim, qm, qc);
+    scan_thread_map<D, REV, R, AFF, Q>(a, r0, r1, start_sim, qm, qc);
     // inclusive scan over the wave: q_l <- q_l o q_{l-s}
 #pragma unroll 1
     for (int s = 1; s < 64; s <<= 1) {
         ss_cd pm[D][D], pc[D];
-        shfl_aff<D, AFF, true>(qm, qc, pm, pc, s);
-        if (lane >= s) aff_before<D, AFF>(qm, qc, pm, pc);
+        shfl_aff<D, AFF, true, Q>(qm, qc, pm, pc, s);
+        if (lane >= s) aff_before<D, AFF, Q>(qm, qc, pm, pc);
     }
     if (lane == 63) {
         ss_copy<D>(qm, s_tm[w]);
@@ -856,7 +895,7 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
     if (R > 1) {
         // exclusive: the rows of earlier lanes of the wave
         ss_cd em[D][D], ec[D];
-        shfl_aff<D, AFF, true>(qm, qc, em, ec, 1);
+        shfl_aff<D, AFF, true, Q>(qm, qc, em, ec, 1);
         if (lane == 0) {
             ss_set_identity<D>(em);
             set_zero_vec<D>(ec);
@@ -871,8 +910,8 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
         ss_set_identity<D>(pm);
         set_zero_vec<D>(pc);
 #pragma unroll 1
-        for (int ww = 0; ww < w; ++ww) aff_after<D, AFF>(s_tm[ww], s_tc[ww], pm, pc);
-        aff_before<D, AFF>(qm, qc, pm, pc);
+        for (int ww = 0; ww < w; ++ww) aff_after<D, AFF, Q>(s_tm[ww], s_tc[ww], pm, pc);
+        aff_before<D, AFF, Q>(qm, qc, pm, pc);
     }
     ss_cd v[D];
     ss_matvec<D>(qm, vin, v);
@@ -906,7 +945,7 @@ __device__ void scan_tile_apply(const ScanArgs& a, long long r0, long long r1, c
     __syncthreads();
 }
 
-template <int D, bool REV, int R>
+template <int D, bool REV, int R, bool Q = false>
 __global__ void __launch_bounds__(kScanThreads) batch_scan_apply(ScanArgs a)
 {
     __shared__ ScanShared sh;
@@ -930,9 +969,9 @@ __global__ void __launch_bounds__(kScanThreads) batch_scan_apply(ScanArgs a)
     for (long long r0 = c0; r0 < c1; r0 += kScanThreads * R) {
         const long long r1 = r0 + kScanThreads * R < c1 ? r0 + kScanThreads * R : c1;
         if (scan_mark_starts<REV, R>(a, r0, r1, start_sim, sh))
-            scan_tile_apply<D, REV, R, true>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
+            scan_tile_apply<D, REV, R, true, Q>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
         else
-            scan_tile_apply<D, REV, R, false>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
+            scan_tile_apply<D, REV, R, false, Q>(a, r0, r1, start_sim, s_tm, s_tc, s_v, vin);
     }
 }
 
@@ -1390,9 +1429,9 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
                             int grid_blocks, int u_layout, ss_stream stream)
 {
     if (!m) return fail("ss_batch_time_evolution: null module");
-    if (u_layout != SS_U_FULL && !(u_layout == SS_U_SU2_COLUMN && m->dim == 2))
-        return fail("ss_batch_time_evolution: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN for spin half)",
-                    u_layout);
+    if (u_layout != SS_U_FULL && !((u_layout == SS_U_SU2_COLUMN || u_layout == SS_U_SU2_FULL) && m->dim == 2))
+        return fail("ss_batch_time_evolution: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN / SS_U_SU2_FULL for "
+                    "spin half)", u_layout);
     if (n_sims < 0 || total < 0) return fail("ss_batch_time_evolution: negative size");
     if (n_sims == 0 || total == 0) return 0;
     if (!sweep || !offsets || !t0 || !dt || !dt_out || !n_fine || !time_out || !u_out || !counter)
@@ -1477,7 +1516,12 @@ int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, int64_t chun
     // bandwidth-bound scans: row-contiguous lanes store coalesced); with
     // kScanRows rows per thread when each chunk is one tile (latency-bound)
     const bool wide = l.chunk_rows > kScanTileRows;
-    if (dim == 2 && !reverse) {
+    if (dim == 2 && !reverse && a.u_layout != SS_U_FULL) {
+        // special unitary operators: maps composed and shuffled as first columns
+        hipLaunchKernelGGL((batch_scan_reduce<2, false, kScanRows, true>), g, b, 0, s, a);
+        if (wide) hipLaunchKernelGGL((batch_scan_apply<2, false, 1, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((batch_scan_apply<2, false, kScanRows, true>), g, b, 0, s, a);
+    } else if (dim == 2 && !reverse) {
         hipLaunchKernelGGL((batch_scan_reduce<2, false, kScanRows>), g, b, 0, s, a);
         if (wide) hipLaunchKernelGGL((batch_scan_apply<2, false, 1>), g, b, 0, s, a);
         else hipLaunchKernelGGL((batch_scan_apply<2, false, kScanRows>), g, b, 0, s, a);
@@ -1582,8 +1626,9 @@ int ss_batch_state_scan(const double* psi0, const double* u, double* state, doub
                         void* workspace, size_t workspace_bytes, ss_stream stream)
 {
     if (dim != 2 && dim != 3) return fail("ss_batch_state_scan: dim must be 2 or 3");
-    if (u_layout != SS_U_FULL && !(u_layout == SS_U_SU2_COLUMN && dim == 2))
-        return fail("ss_batch_state_scan: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN with dim 2)", u_layout);
+    if (u_layout != SS_U_FULL && !((u_layout == SS_U_SU2_COLUMN || u_layout == SS_U_SU2_FULL) && dim == 2))
+        return fail("ss_batch_state_scan: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN / SS_U_SU2_FULL with dim 2)",
+                    u_layout);
     if (n_sims < 0 || total < 0) return fail("ss_batch_state_scan: negative size");
     if (n_sims == 0 || total == 0) return 0;
     if (!psi0 || !u || !state || !offsets) return fail("ss_batch_state_scan: null buffer");

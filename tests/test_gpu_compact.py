@@ -105,3 +105,68 @@ def test_compact_rejected_for_spin_one():
     with pytest.raises(ValueError, match="spin half"):
         ss.BatchPlan(sim, 0.0, 1e-5, 5e-8, 1e-6, ss.SpinQuantumNumber.ONE.plus_z,
                      F.RABI_SWEEP_ONE.sweep, compact=True)
+
+
+def _su2(rng, n):
+    """Random special unitaries [[a, -conj b], [b, conj a]]."""
+    v = rng.normal(size=(n, 4))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    a, b = v[:, 0] + 1j * v[:, 1], v[:, 2] + 1j * v[:, 3]
+    u = np.empty((n, 2, 2), complex)
+    u[:, 0, 0], u[:, 1, 0], u[:, 0, 1], u[:, 1, 1] = a, b, -b.conj(), a.conj()
+    return u
+
+
+@pytest.mark.parametrize("chunk_rows", [0, 1024, 4096])
+def test_su2_scan_layouts_agree(chunk_rows):
+    """ss_batch_state_scan on special unitaries stored full (SS_U_FULL, the general
+    products; SS_U_SU2_FULL, first columns) and as first columns
+    (SS_U_SU2_COLUMN): states and Bloch vectors agree to 1e-12 and match the
+    sequential product; uneven simulations, empty ones, starts inside tiles."""
+    import torch
+
+    from spinsim_amd import _native
+
+    print(f"case: SU(2) scan layouts, chunk_rows={chunk_rows}", flush=True)
+    rng = np.random.default_rng(79)
+    lengths = np.concatenate([rng.integers(0, 9000, 40), [0, 1, 2, 1023, 1024, 1025, 20000]])
+    rng.shuffle(lengths)
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    total, b = int(offsets[-1]), len(lengths)
+    u = _su2(rng, total)
+    psi = rng.normal(size=(b, 2)) + 1j * rng.normal(size=(b, 2))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    L = _native.lib()
+    doff = torch.as_tensor(offsets, device="cuda")
+    dpsi = torch.as_tensor(psi, device="cuda")
+    ws = torch.empty(max(int(L.ss_batch_scan_workspace(total, 2, chunk_rows)), 1), dtype=torch.uint8,
+                     device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    outs = {}
+    for layout, arr in ((_native.U_FULL, u), (_native.U_SU2_FULL, u),
+                        (_native.U_SU2_COLUMN, np.ascontiguousarray(u[:, :, 0]))):
+        du = torch.as_tensor(arr, device="cuda")
+        st = torch.full((total, 2), float("nan"), dtype=torch.complex128, device="cuda")
+        sp = torch.empty((total, 3), dtype=torch.float64, device="cuda")
+        _native.check(L.ss_batch_state_scan(dpsi.data_ptr(), du.data_ptr(), st.data_ptr(), sp.data_ptr(),
+                                            doff.data_ptr(), b, total, 2, chunk_rows, layout,
+                                            ws.data_ptr(), ws.numel(), stream), "scan")
+        outs[layout] = (st.cpu().numpy(), sp.cpu().numpy())
+    ref_st, ref_sp = outs[_native.U_FULL]
+    assert np.isfinite(ref_st).all()
+    for layout in (_native.U_SU2_FULL, _native.U_SU2_COLUMN):
+        st, sp = outs[layout]
+        d_st, d_sp = float(np.abs(st - ref_st).max()), float(np.abs(sp - ref_sp).max())
+        print(f"  layout {layout} vs full products: state {d_st:.2e}, spin {d_sp:.2e}")
+        assert d_st <= 1e-12 and d_sp <= 1e-12
+    # sequential product on a few simulations
+    for s in rng.choice(b, 6, replace=False):
+        a, e = offsets[s], offsets[s + 1]
+        if a == e:
+            continue
+        v = psi[s].copy()
+        seq = [v]
+        for i in range(a + 1, e):
+            v = u[i - 1] @ v
+            seq.append(v)
+        assert np.abs(outs[_native.U_SU2_COLUMN][0][a:e] - np.array(seq)).max() <= 1e-9

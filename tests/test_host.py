@@ -59,7 +59,9 @@ def test_library_rejects_bad_arguments():
     # the first-column operator layout is for spin half only
     assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 3, 0, 1, None, 0, None) != 0
     assert b"u_layout" in lib.ss_last_error()
-    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 2, None, 0, None) != 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 3, 0, 2, None, 0, None) != 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 3, None, 0, None) != 0
+    assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 2, None, 0, None) == 0
     assert lib.ss_batch_state_scan(None, None, None, None, None, 0, 0, 2, 0, 1, None, 0, None) == 0
 
 
