@@ -400,13 +400,19 @@ static __device__ __forceinline__ void ss_call_field(double t, const double* swe
 #endif
 }
 
-// transform_frame_spin_{half,one}_rotating, spinsim/__init__.py:599-613.
+// The transverse field of a sample turns at SS_FRAME_M rot in the rotating
+// frame (spin half: rot is half the midpoint z field, and x + iy turns at
+// twice it), so the windings are kept at that angle, w = exp(i SS_FRAME_M rot
+// t_rel).
+#define SS_FRAME_M (SS_DIM == 2 ? 2.0 : 1.0)
+
+// transform_frame_spin_{half,one}_rotating, spinsim/__init__.py:599-613; w is
+// the sample's winding exp(i SS_FRAME_M rot t_rel).
 static __device__ __forceinline__ void ss_frame(double f[SS_FIELD_LEN], double rot, ss_cd w)
 {
 #if SS_ROT
 #if SS_DIM == 2
-    const ss_cd wc = ss_conj(ss_mul(w, w));
-    const ss_cd x = ss_mul(ss_c(f[0], f[1]), wc);
+    const ss_cd x = ss_mul(ss_c(f[0], f[1]), ss_conj(w));
     f[2] = f[2] - 2.0 * rot;
 #else
     const ss_cd x = ss_mul(ss_c(f[0], f[1]), ss_conj(w));
@@ -790,24 +796,24 @@ static __device__ __forceinline__ void ss_mixed_to_u(const double gsum[SS_LIE], 
 #endif  // SS_DIM
 #endif
 
-// Winding exp(i rot t_rel) of a sample, advanced from step to step by
-// multiplication: w(t') = w(t) exp(i rot dt) (1 + i rot ((t' - t) - dt)), where
+// Winding exp(i r t_rel) of a sample (r = SS_FRAME_M rot), advanced from step
+// to step by multiplication: w(t') = w(t) exp(i r dt) (1 + i r ((t' - t) - dt)), where
 // t' - t is the exact difference of the rounded sample offsets the reference
-// forms, so w follows exp(i rot t_rel) of those offsets to ~1e-15 over a
+// forms, so w follows exp(i r t_rel) of those offsets to ~1e-15 over a
 // chunk; it is re-anchored with sincos at the start of every lane's chunk.
 struct ss_wind {
     ss_cd w;
     double t_rel;
 };
-static __device__ __forceinline__ void ss_wind_advance(ss_wind& s, double t_rel_next, ss_cd step, double rot,
+static __device__ __forceinline__ void ss_wind_advance(ss_wind& s, double t_rel_next, ss_cd step, double r,
                                                        double dt)
 {
 #if SS_ROT
-    const double eps = rot * ((t_rel_next - s.t_rel) - dt);
+    const double eps = r * ((t_rel_next - s.t_rel) - dt);
     s.w = ss_mul(ss_mul(s.w, step), ss_c(1.0, eps));
 #else
     (void)step;
-    (void)rot;
+    (void)r;
     (void)dt;
 #endif
     s.t_rel = t_rel_next;
@@ -843,12 +849,13 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
     if (count <= 0) return;
     ss_wind wa, wb;
     wa.t_rel = (tf + off_a) - tc;
-    wa.w = ss_winding(rot, wa.t_rel);
+    const double frot = SS_FRAME_M * rot;
+    wa.w = ss_winding(frot, wa.t_rel);
     wb.t_rel = (tf + off_b) - tc;
 #if SS_METHOD != 1
-    wb.w = ss_winding(rot, wb.t_rel);
+    wb.w = ss_winding(frot, wb.t_rel);
 #endif
-    const ss_cd step = ss_winding(rot, dt);
+    const ss_cd step = ss_winding(frot, dt);
 #pragma unroll 1
     for (int j = 0; j < count; ++j) {
         double fa[SS_FIELD_LEN], g[SS_FIELD_LEN];
@@ -887,9 +894,9 @@ static __device__ __forceinline__ void ss_integrate(ss_cd u[SS_D][SS_D], int j0,
 #endif
         tf += dt;
         if (j + 1 < count) {
-            ss_wind_advance(wa, (tf + off_a) - tc, step, rot, dt);
+            ss_wind_advance(wa, (tf + off_a) - tc, step, frot, dt);
 #if SS_METHOD != 1
-            ss_wind_advance(wb, (tf + off_b) - tc, step, rot, dt);
+            ss_wind_advance(wb, (tf + off_b) - tc, step, frot, dt);
 #endif
         }
     }
@@ -1573,7 +1580,7 @@ static __device__ __forceinline__ void ss_scatter_sample(const ss_adjoint_args& 
 }
 
 // dL/dg of the rotating-frame samples -> lab frame (x, y rotate back; z, q
-// unchanged); w is the sample's winding exp(i rot t_rel).
+// unchanged); w is the sample's winding exp(i SS_FRAME_M rot t_rel).
 static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], ss_cd w, double out[4])
 {
     out[0] = drot[0];
@@ -1581,12 +1588,7 @@ static __device__ __forceinline__ void ss_unframe(const double drot[SS_LIE], ss_
     out[2] = drot[2];
     out[3] = SS_LIE > 3 ? drot[3] : 0.0;
 #if SS_ROT
-#if SS_DIM == 2
-    const ss_cd wf = ss_mul(w, w);
-#else
-    const ss_cd wf = w;
-#endif
-    const ss_cd gw = ss_mul(ss_c(drot[0], drot[1]), wf);
+    const ss_cd gw = ss_mul(ss_c(drot[0], drot[1]), w);
     out[0] = gw.x;
     out[1] = gw.y;
 #else
@@ -1693,21 +1695,22 @@ ss_batch_adjoint_kernel(ss_adjoint_args a)
     double tf = tc;
 #pragma unroll 1
     for (int k = 0; k + 1 < n_fine; ++k) tf += dt;
-    const ss_cd back = ss_conj(ss_winding(rot, dt));
+    const double frot = SS_FRAME_M * rot;
+    const ss_cd back = ss_conj(ss_winding(frot, dt));
     ss_wind wa, wb;
     wa.t_rel = (tf + off_a) - tc;
-    wa.w = ss_winding(rot, wa.t_rel);
+    wa.w = ss_winding(frot, wa.t_rel);
     wb.t_rel = (tf + off_b) - tc;
 #if SS_METHOD != 1
-    wb.w = ss_winding(rot, wb.t_rel);
+    wb.w = ss_winding(frot, wb.t_rel);
 #endif
 #pragma unroll 1
     for (int j = n_fine - 1; j >= 0; --j) {
         if (j < n_fine - 1) {
             tf -= dt;
-            ss_wind_advance(wa, (tf + off_a) - tc, back, rot, -dt);
+            ss_wind_advance(wa, (tf + off_a) - tc, back, frot, -dt);
 #if SS_METHOD != 1
-            ss_wind_advance(wb, (tf + off_b) - tc, back, rot, -dt);
+            ss_wind_advance(wb, (tf + off_b) - tc, back, frot, -dt);
 #endif
         }
         double fa[SS_FIELD_LEN], g1[SS_FIELD_LEN], d1[SS_LIE], dfa[SS_LIE], lab[4];
