@@ -168,12 +168,35 @@ static __device__ __forceinline__ void ss_square_hypercubes(ss_cd a[SS_D][SS_D])
 
 // ---- matrix exponentials of -i g.J ----------------------------------------
 #if SS_EXPO == 0 && SS_DIM == 2
-// matrix_exponential_analytic, spinsim/__init__.py:1617-1641.
-// cos_exp_m1(r/2, 0) = -2 sin^2(r/4); sin(r/2) from the same sincos.
+// matrix_exponential_analytic, spinsim/__init__.py:1617-1641:
+// exp(-i g.sigma/2) = cos(r/2) I - i (sin(r/2)/r) g.sigma, r = |g|.
+// Both scalars are even in r, so for r < 1/4 they come from their Taylor
+// series in u = r^2/4 (to u^5: truncation below 3e-20 relative) -- no square
+// root, division or normalised axis; beyond, cos_exp_m1(r/2, 0) =
+// -2 sin^2(r/4) and sin(r/2) from one sincos of r/4.
 static __device__ __forceinline__ void ss_expo(const double g[SS_FIELD_LEN], ss_cd e[2][2])
 {
     double x = g[0], y = g[1], z = g[2];
-    const double r = sqrt(x * x + y * y + z * z);
+    const double r2 = x * x + y * y + z * z;
+    if (r2 < 0.0625) {
+        const double u = 0.25 * r2;
+        // cos(r/2) - 1 and sin(r/2)/r
+        const double c = u * fma(u, fma(u, fma(u, fma(u, -2.755731922398589e-07, 2.48015873015873e-05),
+                                                -0.001388888888888889),
+                                        0.041666666666666664),
+                                 -0.5);
+        const double sr = 0.5 * fma(u, fma(u, fma(u, fma(u, fma(u, -2.505210838544172e-08, 2.7557319223985893e-06),
+                                                         -0.0001984126984126984),
+                                                 0.008333333333333333),
+                                         -0.16666666666666666),
+                                    1.0);
+        e[0][0] = ss_c(c + 1.0, -z * sr);
+        e[1][0] = ss_c(y * sr, -x * sr);
+        e[0][1] = ss_c(-y * sr, -x * sr);
+        e[1][1] = ss_c(c + 1.0, z * sr);
+        return;
+    }
+    const double r = sqrt(r2);
     if (r > 0) {
         const double ir = 1.0 / r;
         x *= ir;
