@@ -20,7 +20,9 @@ the 4096-simulation spin-half Rabi ensemble with uneven durations (1-50 ms,
 float32-bulk, fp32 and float64 graphs (one persistent launch + batch state
 scan, HIP-graph replay) and through a loop of one plain float64 evaluate per
 simulation (the reference's way through a sweep), each checked on 64 sampled
-simulations against the float64 oracle, with the batched/plain ratios; the
+simulations against the float64 oracle, with the batched/plain ratios, and
+the batched fp32 and float64 graphs again with operators stored as first
+columns (states and Bloch vectors only); the
 differentiation workload (1024 spin-one pulses, forward + adjoint gradient,
 float64 and float32-bulk forward, and the deterministic gradient reduction),
 with its gradient error against central differences; the long noisy spin-half
