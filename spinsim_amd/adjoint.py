@@ -58,8 +58,11 @@ class SsControlDesc(ctypes.Structure):
 def _combined_source(static, controls):
     """HIP-C text: the static field (renamed) plus the control terms; it sets
     SS_ADJ_CONTROLS, the adjoint kernel's number of per-lane gradient moment
-    sums, to the control count."""
-    parts = [f"#define SS_ADJ_CONTROLS {len(controls)}\n"]
+    sums, to the control count, and SS_ADJ_AXES to the set of their axes."""
+    axes = 0
+    for c in controls:
+        axes |= 1 << c.axis
+    parts = [f"#define SS_ADJ_CONTROLS {len(controls)}\n#define SS_ADJ_AXES {axes}u\n"]
     if static is not None:
         parts.append("#define get_field ss_static_field\n" + static.source + "\n#undef get_field\n")
     call = ""
@@ -206,6 +209,9 @@ class ControlledSimulator:
             raise RuntimeError("ss_batch_adjoint: a coarse step's samples reach beyond its "
                                "8-knot gradient window, which the deterministic mode needs; "
                                "use a shorter time_step_output or deterministic=False")
+        if flags & 4:
+            raise RuntimeError("ss_batch_adjoint: a control's axis is not among those the "
+                               "module was compiled for (SS_ADJ_AXES)")
         return grad
 
 

@@ -1216,8 +1216,8 @@ extern "C" __global__ void SS_KERNEL_ATTR ss_batch_time_evolution_kernel(ss_batc
 // ===========================================================================
 #define SS_ADJ_ORDER 5       // Frechet series: terms p + q <= 5
 #define SS_ADJ_TAYLOR 7      // Taylor order of the exponentials walking psi and lambda back
-#define SS_ADJ_NU 0.035      // sub-step until the generator norm bound is below this
-#define SS_ADJ_MAX_SPLIT 10  // at most 2^10 sub-steps (norm bound up to 35.8)
+#define SS_ADJ_NU 0.024      // sub-step until the generator norm bound is below this
+#define SS_ADJ_MAX_SPLIT 11  // at most 2^11 sub-steps (norm bound up to 49)
 
 struct ss_gen64 {
     double d[SS_D];
@@ -1307,73 +1307,84 @@ static __device__ __forceinline__ void ss_grad_dots(const ss_cd gam[SS_D], const
 }
 
 // One piece walked backward: psi <- e^{-A} psi, lambda <- e^{-A} lambda
-// (Taylor series to order SS_ADJ_TAYLOR: below nu^8 / 8! = 5.6e-17 relative
+// (Taylor series to order SS_ADJ_TAYLOR: below nu^8 / 8! = 2.7e-18 relative
 // at the sub-step bound), and grad[k] += w Re(lambda^+ L(A, E_k) psi_before)
-// for the axes in mask, L(A, E) = sum_{p,q} A^p E A^q / (p + q + 1)!:
-//     Re(lambda^+ L(A, E) psi_before) = sum_{p+q<=N} c_{p+q} Re(alpha_p^+ E beta_q),
-// alpha_p = (A^+)^p lambda = (-A)^p lambda (kept: they are also the terms of
-// lambda's own series), beta_q = A^q psi_before (streamed), c_n = 1/(n+1)!,
-// and gamma_q = sum_p c_{p+q} alpha_p formed for each q.  The series left
-// out (p + q > N = SS_ADJ_ORDER) is below nu^(N+1) / (N+1)! of |E|
-// relative: 2.5e-12 at nu = 0.035, 6.7e-15 at the bench workload's 0.013.
+// for the axes in mask, psi_before = e^{-A} psi.  With
+// L(A, E) psi_before = int_0^1 e^{sA} E e^{(1-s)A} psi_before ds
+//                    = int_0^1 e^{sA} E e^{-sA} psi ds
+// and both exponentials expanded in s,
+//     Re(lambda^+ L(A, E) psi_before) = sum_{p+q<=N} Re(alpha_p^+ E b_q) / (p! q! (p+q+1)),
+// alpha_p = (A^+)^p lambda = (-A)^p lambda and b_q = (-A)^q psi: the terms of
+// lambda's and psi's own series, so the derivative takes no operator
+// products of its own: gamma_q = sum_p alpha_p / (p! q! (p+q+1)) is summed
+// as lambda's series streams alpha_p, then dotted with each b_q as psi's
+// series streams it.  The terms
+// of order p + q = n sum to lambda^+ ad_A^n(E) psi / (n! (n+1)), ad_A(E) =
+// AE - EA with norm at most 2 nu |E|, so the
+// series left out (p + q > N = SS_ADJ_ORDER) is below
+// (2 nu)^(N+1) / ((N+1)! (N+2)) of |E|: 2.4e-12 at the sub-step bound
+// nu = 0.024, 6e-14 at the bench workload's 0.013.
+// An empty asm statement that reads and rewrites v: v is complete here, and
+// what uses it afterwards starts here (the statements keep their order).
+static __device__ __forceinline__ void ss_pin(ss_cd v[SS_D])
+{
+#pragma unroll
+    for (int r = 0; r < SS_D; ++r) asm volatile("" : "+v"(v[r].x), "+v"(v[r].y));
+}
+
 static __device__ __forceinline__ void ss_adjoint_piece(const ss_gen64& na, ss_cd psi[SS_D], ss_cd lam[SS_D],
                                                         unsigned mask, double w, double grad[SS_LIE])
 {
-    // psi <- e^{-A} psi = sum_q na^q psi / q!
+    // lambda <- sum_p alpha_p / p!, alpha_p = na^p lambda streamed, each
+    // alpha_p (p <= N) added into gamma_q for q <= N - p
+    ss_cd gam[SS_ADJ_ORDER + 1][SS_D];
     {
-        ss_cd t[SS_D], t2[SS_D], acc[SS_D];
+        ss_cd al[SS_D], acc[SS_D];
 #pragma unroll
-        for (int r = 0; r < SS_D; ++r) t[r] = acc[r] = psi[r];
+        for (int r = 0; r < SS_D; ++r) {
+            al[r] = acc[r] = lam[r];
 #pragma unroll
-        for (int q = 1; q <= SS_ADJ_TAYLOR; ++q) {
-            ss_gen_vec(na, t, t2);
+            for (int q = 0; q <= SS_ADJ_ORDER; ++q) gam[q][r] = ss_scale(lam[r], ss_inv_fact[q] / (q + 1));
+        }
 #pragma unroll
-            for (int r = 0; r < SS_D; ++r) t[r] = t2[r];
-            ss_axpy(acc, ss_inv_fact[q], t);
+        for (int p = 1; p <= SS_ADJ_TAYLOR; ++p) {
+            ss_cd t[SS_D];
+            ss_gen_vec(na, al, t);
+#pragma unroll
+            for (int r = 0; r < SS_D; ++r) al[r] = t[r];
+            ss_axpy(acc, ss_inv_fact[p], al);
+#pragma unroll
+            for (int q = 0; q <= SS_ADJ_ORDER - p; ++q)
+                ss_axpy(gam[q], ss_inv_fact[p] * ss_inv_fact[q] / (p + q + 1), al);
+        }
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) lam[r] = acc[r];
+    }
+#pragma unroll
+    for (int q = 0; q <= SS_ADJ_ORDER; ++q) ss_pin(gam[q]);
+    // (left to itself the compiler interleaves the two streams, keeping the
+    // powers alpha_p and the sums gamma_q live at once: 298 registers where
+    // two waves per SIMD have 256, and the kernel spills)
+    ss_pin(lam);
+    ss_pin(psi);
+    // psi <- sum_q b_q / q!, b_q = na^q psi streamed, with the dots of gamma_q
+    {
+        ss_cd bq[SS_D], acc[SS_D];
+#pragma unroll
+        for (int r = 0; r < SS_D; ++r) bq[r] = acc[r] = psi[r];
+#pragma unroll
+        for (int q = 0; q <= SS_ADJ_TAYLOR; ++q) {
+            if (q > 0) {
+                ss_cd t[SS_D];
+                ss_gen_vec(na, bq, t);
+#pragma unroll
+                for (int r = 0; r < SS_D; ++r) bq[r] = t[r];
+                ss_axpy(acc, ss_inv_fact[q], bq);
+            }
+            if (q <= SS_ADJ_ORDER) ss_grad_dots(gam[q], bq, mask, w, grad);
         }
 #pragma unroll
         for (int r = 0; r < SS_D; ++r) psi[r] = acc[r];
-    }
-    // alpha_p = na^p lambda
-    ss_cd al[SS_ADJ_ORDER + 1][SS_D];
-#pragma unroll
-    for (int r = 0; r < SS_D; ++r) al[0][r] = lam[r];
-#pragma unroll
-    for (int p = 1; p <= SS_ADJ_ORDER; ++p) ss_gen_vec(na, al[p - 1], al[p]);
-    // beta_q = A^q psi_before = (-1)^q na^q psi, streamed
-    ss_cd b[SS_D];
-#pragma unroll
-    for (int r = 0; r < SS_D; ++r) b[r] = psi[r];
-#pragma unroll
-    for (int q = 0; q <= SS_ADJ_ORDER; ++q) {
-        if (q > 0) {
-            ss_cd t[SS_D];
-            ss_gen_vec(na, b, t);
-#pragma unroll
-            for (int r = 0; r < SS_D; ++r) b[r] = t[r];
-        }
-        const double sign = (q & 1) ? -1.0 : 1.0;
-        ss_cd gam[SS_D];
-#pragma unroll
-        for (int r = 0; r < SS_D; ++r) gam[r] = ss_scale(al[0][r], sign * ss_inv_fact[q + 1]);
-#pragma unroll
-        for (int p = 1; p <= SS_ADJ_ORDER - q; ++p) ss_axpy(gam, sign * ss_inv_fact[p + q + 1], al[p]);
-        ss_grad_dots(gam, b, mask, w, grad);
-    }
-    // lambda <- sum_p alpha_p / p!, the orders past N continued from alpha_N
-    {
-        ss_cd t[SS_D], t2[SS_D];
-#pragma unroll
-        for (int r = 0; r < SS_D; ++r) t[r] = al[SS_ADJ_ORDER][r];
-#pragma unroll
-        for (int p = 1; p <= SS_ADJ_ORDER; ++p) ss_axpy(lam, ss_inv_fact[p], al[p]);
-#pragma unroll
-        for (int p = SS_ADJ_ORDER + 1; p <= SS_ADJ_TAYLOR; ++p) {
-            ss_gen_vec(na, t, t2);
-#pragma unroll
-            for (int r = 0; r < SS_D; ++r) t[r] = t2[r];
-            ss_axpy(lam, ss_inv_fact[p], t);
-        }
     }
 }
 
@@ -1515,6 +1526,15 @@ static __device__ __forceinline__ void ss_scatter_knots(const ss_adjoint_args& a
 #define SS_ADJ_CONTROLS SS_MAX_CONTROLS
 #endif
 
+// A field source may also define SS_ADJ_AXES, the set of its controls' axes
+// (bit k: axis k): the derivatives the kernel forms are then known when it is
+// compiled, and those of other axes take no registers or instructions.  A
+// control on an axis outside the set sets status bit 3 (value 4) and its row
+// adds nothing.
+#ifndef SS_ADJ_AXES
+#define SS_ADJ_AXES 15u
+#endif
+
 static __device__ __forceinline__ double* ss_moment_slot(double* win, int c, int m)
 {
     return win + (c * SS_ADJ_SLOTS + SS_ADJ_WINDOW + m) * SS_ADJ_BLOCK;
@@ -1653,6 +1673,10 @@ ss_batch_adjoint_kernel(ss_adjoint_args a)
     long long base[SS_MAX_CONTROLS];
     for (int c = 0; c < a.n_controls; ++c) {
         const int ax = a.controls[c].axis;
+        if (!((SS_ADJ_AXES >> ax) & 1u)) {
+            atomicOr(a.status, 4);
+            return;
+        }
         mask |= (ax <= 1 && SS_ROT) ? 3u : (1u << ax);
         z_control = z_control || ax == 2;
         // the window starts one knot before the interval holding tc; past the
@@ -1666,9 +1690,14 @@ ss_batch_adjoint_kernel(ss_adjoint_args a)
 #if SS_ROT
     // the frame term of a z control needs dL/dg of x, y and z
     if (z_control) mask |= 7u;
+    constexpr unsigned axes_mask = ((SS_ADJ_AXES & 3u) ? 3u : 0u) | ((SS_ADJ_AXES & 4u) ? 7u : 0u) | (SS_ADJ_AXES & 8u);
 #else
     (void)z_control;
+    constexpr unsigned axes_mask = SS_ADJ_AXES;
 #endif
+    // (bits the compiler can see are clear)
+    mask &= axes_mask;
+    z_control = z_control && (SS_ADJ_AXES & 4u);
 
     long long mo[SS_ADJ_CONTROLS];  // interval of each control's moment sums, -1: none
 #pragma unroll

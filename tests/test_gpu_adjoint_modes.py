@@ -6,6 +6,7 @@
 * Four controls (x, y, z, q on spin one; the largest LDS window launch)
   against the float64 CPU reference gradient and central differences, in
   both modes.
+* A control whose axis the module was not compiled for is refused.
 """
 from __future__ import annotations
 
@@ -130,3 +131,29 @@ def test_four_controls_spin_one(deterministic):
     rel = np.abs(fd - ad).max() / np.abs(ad).max()
     print(f"  {len(coords)} coordinates vs central differences: {rel:.3e}")
     assert rel < 1e-5
+
+
+def test_control_axis_outside_compiled_set_is_refused():
+    """A module compiled for x and y controls (SS_ADJ_AXES = 3) refuses a
+    descriptor naming z at launch: every row returns early with status bit 3."""
+    import torch
+
+    import spinsim_amd as ss
+
+    print("case: control axis outside the compiled set", flush=True)
+    rng = np.random.default_rng(109)
+    t1 = 5e-6
+    tk = np.linspace(0.0, t1, 11)
+    amp = rng.normal(0, TAU * 5e4, (2, 2, tk.size))
+    cs = ss.ControlledSimulator(ss.SpinQuantumNumber.ONE, [ss.Control("x", tk), ss.Control("y", tk)],
+                                _static_z_one)
+    psi0 = ss.SpinQuantumNumber.ONE.plus_z
+    g, _ = _grad(cs, amp, t1, psi0, False)  # compiles the module for axes {x, y}
+    assert np.isfinite(g).all()
+    cs.controls[1].axis = 2  # the descriptors now name z; the module stays compiled for x, y
+    try:
+        with pytest.raises(RuntimeError, match="SS_ADJ_AXES"):
+            _grad(cs, amp, t1, psi0, False)
+    finally:
+        cs.controls[1].axis = 1
+    torch.cuda.synchronize()
