@@ -1061,10 +1061,10 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
 // is 512 / waves VGPRs a lane (in 8-register granules).  The time-evolution
 // kernels take what they need (about 230-260 for spin one: their float64
 // chains keep one or two waves per SIMD busy); the adjoint kernel is held to
-// 256, two waves per SIMD, to hide its table and LDS latencies (8.4 ms
-// against 11.3 ms at 270 VGPRs and one wave, for 1024 spin-one pulses of
-// 1e5 steps, at the cost of a few spilled registers).  A field source may
-// define either to retune.
+// 256, two waves per SIMD, to hide its table and LDS latencies (backward
+// pass 5.9 ms against 8.1 ms at 266 VGPRs and one wave, for 1024 spin-one
+// pulses of 1e5 steps, at the cost of a few spilled registers of its cold
+// paths).  A field source may define either to retune.
 #ifndef SS_WAVES_PER_EU
 #define SS_WAVES_PER_EU 1
 #endif
