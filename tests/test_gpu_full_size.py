@@ -3,7 +3,8 @@
 * The 4096-simulation spin-half Rabi ensemble (durations 1-50 ms, 50 ns
   steps, 1 us outputs: 1.06e8 coarse rows, a 6.8 GB operator array, so rows
   late in the CSR range sit beyond 2^32 bytes), float64, float32-bulk and fp32,
-  in one persistent launch and one batch scan.  64 simulations -- the first,
+  in one persistent launch and one batch scan; float64 and fp32 also with the
+  operators kept as first columns (compact).  64 simulations -- the first,
   the last, the longest, the shortest and the rest spread over the range --
   are checked against the C oracle (float64 1e-9, float32 modes 1e-6), and
   the last simulation's operators and Bloch vectors too.
@@ -50,18 +51,25 @@ def _checked(dur, count=64):
     return sorted(picks)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32_bulk", 1e-6), ("fp32", 1e-6)])
-def test_ensemble_4096_full_size(precision, tol):
+@pytest.mark.parametrize("precision,tol,compact", [("fp64", 1e-9, False), ("fp32_bulk", 1e-6, False),
+                                                   ("fp32", 1e-6, False), ("fp64", 1e-9, True),
+                                                   ("fp32", 1e-6, True)],
+                         ids=["fp64", "fp32_bulk", "fp32", "fp64_compact", "fp32_compact"])
+def test_ensemble_4096_full_size(precision, tol, compact):
     import torch
 
     import spinsim_amd as ss
+    from spinsim_amd.batch import su2_from_columns
 
-    print(f"case: 4096-simulation ensemble, {precision}", flush=True)
+    print(f"case: 4096-simulation ensemble, {precision}, compact={compact}", flush=True)
     sweep, dur = _ensemble()
     psi0 = ss.SpinQuantumNumber.HALF.plus_z
     sim = ss.Simulator(RABI_SWEEP, ss.SpinQuantumNumber.HALF, precision=precision)
-    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep, with_spin=True)
-    assert plan.total > 1e8 and plan.u.numel() * 16 > 2 ** 32
+    plan = ss.BatchPlan(sim, 0.0, dur, 50e-9, 1e-6, psi0, sweep, with_spin=True, compact=compact)
+    row_bytes = 32 if compact else 64
+    assert plan.total > 1e8 and plan.u.numel() * 16 == plan.total * row_bytes
+    if not compact:
+        assert plan.u.numel() * 16 > 2 ** 32
     plan.run()
     torch.cuda.synchronize()
     co = COracle(RABI_SWEEP, 2)
@@ -76,10 +84,11 @@ def test_ensemble_4096_full_size(precision, tol):
         err = float(np.abs(st - st_ref).max())
         worst = max(worst, err)
         assert err <= tol, f"simulation {s}: state error {err:.3e}"
-        if s == len(dur) - 1:  # the last rows of the arrays, beyond 4 GiB of U
-            assert a * 64 > 2 ** 32
+        if s == len(dur) - 1:  # the last rows of the arrays (beyond 4 GiB of a full U)
+            assert a * row_bytes > 2 ** 32 or compact
             assert np.array_equal(plan.time[a:b].cpu().numpy(), t_ref)
-            u_err = float(np.abs(plan.u[a:b].cpu().numpy() - u_ref).max())
+            u = (su2_from_columns(plan.u[a:b]) if compact else plan.u[a:b]).cpu().numpy()
+            u_err = float(np.abs(u - u_ref).max())
             assert u_err <= tol, f"last simulation: operator error {u_err:.3e}"
             spin = plan.spin_out[a:b].cpu().numpy()
             assert np.abs(spin - co.get_spin(st_ref)).max() <= tol
