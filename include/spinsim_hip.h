@@ -126,7 +126,9 @@ uint64_t ss_source_hash(void);
  * or, with field_has_context = 1, the same with a trailing SS_CONTEXT
  * argument (for samplers).  The text may also define SS_ADJ_CONTROLS (1..4,
  * the number of controls ss_batch_adjoint will be given: per-lane state for
- * that many; default 4), SS_WAVES_PER_EU or SS_ADJ_WAVES_PER_EU (the waves
+ * that many; default 4), SS_ADJ_AXES (the set of their axes, bit k for axis
+ * k: derivatives of other axes are compiled out; default 15, all four),
+ * SS_WAVES_PER_EU or SS_ADJ_WAVES_PER_EU (the waves
  * per SIMD the time-evolution and adjoint kernels are compiled for; defaults
  * 1 and 2).  dim is 2 (spin half) or 3 (spin one);
  * number_of_squares as in the reference (Lie-Trotter squarings, :1547-1550);
@@ -248,7 +250,9 @@ int ss_batch_final_state(const double *psi0, const double *u, const double *stat
  * axis) and owns gradient entries [grad_offset, grad_offset + that table's
  * size) of each row; entries at or beyond grad_stride are dropped.  grad
  * (n_sims, grad_stride) and *status are cleared by the call; status bit 1 is
- * set if a step's generator was too large for the derivative series.
+ * set if a step's generator was too large for the derivative series, status
+ * bit 4 if a control's axis is outside the module's SS_ADJ_AXES (its rows
+ * then add nothing).
  * workspace NULL: contributions are summed with float64 atomics, so the
  * rounding of a gradient entry can differ from run to run (sums of the same
  * terms in another order).  workspace of ss_batch_adjoint_workspace(total,

@@ -1529,8 +1529,8 @@ static __device__ __forceinline__ void ss_scatter_knots(const ss_adjoint_args& a
 // A field source may also define SS_ADJ_AXES, the set of its controls' axes
 // (bit k: axis k): the derivatives the kernel forms are then known when it is
 // compiled, and those of other axes take no registers or instructions.  A
-// control on an axis outside the set sets status bit 3 (value 4) and its row
-// adds nothing.
+// control on an axis outside the set sets status bit 4 and its row adds
+// nothing.
 #ifndef SS_ADJ_AXES
 #define SS_ADJ_AXES 15u
 #endif

@@ -135,7 +135,7 @@ def test_four_controls_spin_one(deterministic):
 
 def test_control_axis_outside_compiled_set_is_refused():
     """A module compiled for x and y controls (SS_ADJ_AXES = 3) refuses a
-    descriptor naming z at launch: every row returns early with status bit 3."""
+    descriptor naming z at launch: every row returns early with status bit 4."""
     import torch
 
     import spinsim_amd as ss
