@@ -1517,8 +1517,20 @@ int launch_scan(const char* who, ScanArgs a, int dim, bool reverse, int64_t chun
     // kScanRows rows per thread when each chunk is one tile (latency-bound)
     const bool wide = l.chunk_rows > kScanTileRows;
     if (dim == 2 && !reverse && a.u_layout != SS_U_FULL) {
-        // special unitary operators: maps composed and shuffled as first columns
-        hipLaunchKernelGGL((batch_scan_reduce<2, false, kScanRows, true>), g, b, 0, s, a);
+        // special unitary operators: maps composed and shuffled as first
+        // columns.  Pass 1 over wide chunks reads 64 bytes of rows per lane
+        // (two first-column rows, or one full row whose first column it
+        // reads): the lanes' loads then share cache lines, where four rows a
+        // lane (128 or 256 bytes apart) left pass 1 bound by the load
+        // requests -- 1.02 -> 0.83 ms (first columns) and 1.52 -> 1.28 ms
+        // (full rows) for 1.06e8 rows; spin one keeps four rows a lane (one
+        // a lane: 2.2 -> 2.7 ms for its scans of 2e7 rows)
+        if (!wide)
+            hipLaunchKernelGGL((batch_scan_reduce<2, false, kScanRows, true>), g, b, 0, s, a);
+        else if (a.u_layout == SS_U_SU2_COLUMN)
+            hipLaunchKernelGGL((batch_scan_reduce<2, false, 2, true>), g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL((batch_scan_reduce<2, false, 1, true>), g, b, 0, s, a);
         if (wide) hipLaunchKernelGGL((batch_scan_apply<2, false, 1, true>), g, b, 0, s, a);
         else hipLaunchKernelGGL((batch_scan_apply<2, false, kScanRows, true>), g, b, 0, s, a);
     } else if (dim == 2 && !reverse) {
