@@ -260,6 +260,35 @@ __device__ static inline double ss_sin32(double x)
     return fma(ca, fma(b, sm, b), fma(sa, cm, sa));
 }
 
+// ---- float64 sin and cos for field callbacks (precisions other than fp32) ---
+// The reduction and table of ss_trig_reduce (x = k h + b, |b| <= h / 2 =
+// pi / 256), then cos b - 1 and sin b / b - 1 by their Taylor series to b^6
+// in float64 (the first terms left out, b^8/8! and b^8/9!, are below 2e-20):
+//   cos x = cos a + cos a (cos b - 1) - sin a b (1 + (sin b / b - 1)).
+// Arguments beyond the reduction's range (and NaN) take the library functions.
+__device__ static inline void ss_trig_small64(double b, double* cm, double* sm)
+{
+    const double b2 = b * b;
+    *cm = b2 * fma(b2, fma(b2, -1.0 / 720.0, 1.0 / 24.0), -0.5);
+    *sm = b2 * fma(b2, fma(b2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0);
+}
+
+__device__ static inline double ss_cos64(double x)
+{
+    double ca, sa, b, cm, sm;
+    if (!ss_trig_reduce(x, &ca, &sa, &b)) return cos(x);
+    ss_trig_small64(b, &cm, &sm);
+    return fma(-sa, fma(b, sm, b), fma(ca, cm, ca));
+}
+
+__device__ static inline double ss_sin64(double x)
+{
+    double ca, sa, b, cm, sm;
+    if (!ss_trig_reduce(x, &ca, &sa, &b)) return sin(x);
+    ss_trig_small64(b, &cm, &sm);
+    return fma(ca, fma(b, sm, b), fma(sa, cm, sa));
+}
+
 // 2^(j/64), j = 0 .. 63
 __constant__ static const double ss_exp_table[64] = {
     1.0, 1.0108892860517005, 1.0218971486541166, 1.0330248790212284,

@@ -117,4 +117,15 @@ int ss_host_f32_eval(int fn, const double* x, double* out, long long n)
     return 0;
 }
 
+// The field callbacks' table-driven float64 sin and cos (every precision but
+// fp32) at n points: fn 0 sin, 1 cos (ss_sin64 / ss_cos64 of ss_prelude.h).
+int ss_host_f64_eval(int fn, const double* x, double* out, long long n)
+{
+    if (n < 0 || fn < 0 || fn > 1) return 1;
+    if (n == 0) return 0;
+    if (!x || !out) return 1;
+    for (long long i = 0; i < n; ++i) out[i] = fn == 0 ? ss_sin64(x[i]) : ss_cos64(x[i]);
+    return 0;
+}
+
 }  // extern "C"

@@ -1145,12 +1145,15 @@ std::string build_source(const char* field_src, int dim, int method, int expo, i
     s += ss_src_args;
     s += ss_src_complex;
     s += ss_src_prelude;
-    // precision fp32: the callback's sin, cos and exp in float32 (ss_prelude.h)
+    // the callback's sin and cos: the table-driven float64 functions, or with
+    // precision fp32 float32 ones with exp as well (ss_prelude.h)
     const bool f32 = precision == SS_PRECISION_FP32;
     if (f32) s += "\n#define sin(x) ss_sin32(x)\n#define cos(x) ss_cos32(x)\n#define exp(x) ss_exp32(x)\n";
+    else s += "\n#define sin(x) ss_sin64(x)\n#define cos(x) ss_cos64(x)\n";
     s += "\n#line 1 \"get_field\"\n";
     s += field_src;
     if (f32) s += "\n#undef sin\n#undef cos\n#undef exp\n";
+    else s += "\n#undef sin\n#undef cos\n";
     s += "\n#line 1 \"ss_integrator.h\"\n";
     s += ss_src_integrator;
     return s;

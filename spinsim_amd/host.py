@@ -94,6 +94,8 @@ def host_lib():
             lib.ss_host_interp_eval.restype = c_int
             lib.ss_host_f32_eval.argtypes = [c_int, c_vp, c_vp, c_ll]
             lib.ss_host_f32_eval.restype = c_int
+            lib.ss_host_f64_eval.argtypes = [c_int, c_vp, c_vp, c_ll]
+            lib.ss_host_f64_eval.restype = c_int
             _host_lib = lib
     return _host_lib
 
@@ -130,14 +132,17 @@ def module_source(field_src, dim, method, expo, rotating, number_of_squares, has
             f"#define SS_EXPO {expo}\n#define SS_ROT {1 if rotating else 0}\n"
             f"#define SS_HYPER {hyper}\n#define SS_CTX {1 if has_context else 0}\n"
             f"#define SS_PREC {precision}\n")
-    # precision fp32: the callback's sin, cos and exp in float32 (ss_prelude.h)
-    f32_on = "#define sin(x) ss_sin32(x)\n#define cos(x) ss_cos32(x)\n#define exp(x) ss_exp32(x)\n"
-    f32_off = "#undef sin\n#undef cos\n#undef exp\n"
-    f32 = precision == 3
+    # the callback's sin and cos: the table-driven float64 functions, or with
+    # precision fp32 float32 ones with exp as well (ss_prelude.h), as on the GPU
+    if precision == 3:
+        on = "#define sin(x) ss_sin32(x)\n#define cos(x) ss_cos32(x)\n#define exp(x) ss_exp32(x)\n"
+        off = "#undef sin\n#undef cos\n#undef exp\n"
+    else:
+        on = "#define sin(x) ss_sin64(x)\n#define cos(x) ss_cos64(x)\n"
+        off = "#undef sin\n#undef cos\n"
     return (head + '#include "host/ss_host_prelude.h"\n#include "device/ss_args.h"\n'
             '#include "device/ss_complex.h"\n#include "device/ss_prelude.h"\n'
-            + (f32_on if f32 else "") + '#line 1 "get_field"\n' + field_src + "\n"
-            + (f32_off if f32 else "") +
+            + on + '#line 1 "get_field"\n' + field_src + "\n" + off +
             '#include "device/ss_integrator.h"\n#include "host/ss_host_module.h"\n')
 
 
@@ -227,6 +232,16 @@ def f32_eval(fn, x):
     out = np.empty_like(x)
     _check(host_lib().ss_host_f32_eval(("sin", "cos", "exp").index(fn), x.ctypes.data,
                                        out.ctypes.data, x.size), "ss_host_f32_eval")
+    return out
+
+
+def f64_eval(fn, x):
+    """The field callbacks' table-driven float64 sin / cos (every precision
+    but "fp32"), host build: fn in ("sin", "cos")."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, np.float64)))
+    out = np.empty_like(x)
+    _check(host_lib().ss_host_f64_eval(("sin", "cos").index(fn), x.ctypes.data,
+                                       out.ctypes.data, x.size), "ss_host_f64_eval")
     return out
 
 
