@@ -1151,18 +1151,29 @@ extern "C" __global__ void SS_KERNEL_ATTR ss_batch_time_evolution_kernel(ss_batc
         if (lane == 0) start = atomicAdd(a.counter, (unsigned long long)grab);
         start = __shfl(start, 0, 64);
         if ((long long)start >= a.total) break;
-#pragma unroll 1
-        for (int it = 0; it < a.items_per_grab; ++it) {
-            const long long row = (long long)start + (long long)it * groups + gi;
-            const bool active = gi < groups && row < a.total;
-            // simulation owning `row`: largest s with offsets[s] <= row
-            long long lo = 0, hi = a.n_sims;
-            if (active) {
+        // simulation owning the lane's first row of the grab: largest s with
+        // offsets[s] <= row, by bisection; the lane's later rows move it on
+        // by the few starts they pass (a chain of dependent loads per grab
+        // instead of per item)
+        long long lo = 0;
+        {
+            const long long row = (long long)start + gi;
+            long long hi = a.n_sims;
+            if (gi < groups && row < a.total) {
 #pragma unroll 1
                 while (hi - lo > 1) {
                     const long long mid = (lo + hi) >> 1;
                     if (a.offsets[mid] <= row) lo = mid; else hi = mid;
                 }
+            }
+        }
+#pragma unroll 1
+        for (int it = 0; it < a.items_per_grab; ++it) {
+            const long long row = (long long)start + (long long)it * groups + gi;
+            const bool active = gi < groups && row < a.total;
+            if (active) {
+#pragma unroll 1
+                while (lo + 1 < a.n_sims && a.offsets[lo + 1] <= row) ++lo;
             }
             const int sim = (int)lo;
             const long long local = active ? row - a.offsets[sim] : 0;
