@@ -33,7 +33,7 @@
 // sin and cos of x.  The Lie-Trotter generators are scaled by 4^-hypercubes,
 // so their angles are tiny; there the Taylor series to x^9 / x^10 is exact to
 // double precision (|x| < 1/16: truncation below 3e-20 relative).  Larger
-// arguments go to the device libm.
+// arguments take the table-driven ss_sincos64.
 static __device__ __forceinline__ void ss_sincos(double x, double* s, double* c)
 {
     if (fabs(x) < 0.0625) {
@@ -51,7 +51,7 @@ static __device__ __forceinline__ void ss_sincos(double x, double* s, double* c)
                      -0.5),
                  1.0);
     } else {
-        sincos(x, s, c);
+        ss_sincos64(x, s, c);
     }
 }
 
@@ -465,7 +465,7 @@ static __device__ __forceinline__ ss_cd ss_winding(double rot, double t_rel)
 {
 #if SS_ROT
     double s, c;
-    sincos(rot * t_rel, &s, &c);
+    ss_sincos64(rot * t_rel, &s, &c);
     return ss_c(c, s);
 #else
     (void)rot;
@@ -1027,7 +1027,7 @@ static __device__ __forceinline__ void ss_store(ss_cd u[SS_D][SS_D], double rot,
 #endif
 #if SS_ROT
     double s, c;
-    sincos(rot * dt_out, &s, &c);
+    ss_sincos64(rot * dt_out, &s, &c);
     const ss_cd w = ss_c(c, s);
     const ss_cd wc = ss_conj(w);
 #pragma unroll
@@ -1735,7 +1735,7 @@ ss_batch_adjoint_kernel(ss_adjoint_args a)
     {
         // U = X P with X = diag(conj(w), 1, w): psi_rot = X^+ psi, lam_rot = X^+ lam
         double s, c;
-        sincos(rot * dt_out, &s, &c);
+        ss_sincos64(rot * dt_out, &s, &c);
         const ss_cd w = ss_c(c, s);
         psi[0] = ss_mul(psi[0], w);
         lam[0] = ss_mul(lam[0], w);

@@ -289,6 +289,20 @@ __device__ static inline double ss_sin64(double x)
     return fma(ca, fma(b, sm, b), fma(sa, cm, sa));
 }
 
+// sin and cos of x together: one reduction, one table load.
+__device__ static inline void ss_sincos64(double x, double* s, double* c)
+{
+    double ca, sa, b, cm, sm;
+    if (!ss_trig_reduce(x, &ca, &sa, &b)) {
+        sincos(x, s, c);
+        return;
+    }
+    ss_trig_small64(b, &cm, &sm);
+    const double bs = fma(b, sm, b);
+    *s = fma(ca, bs, fma(sa, cm, sa));
+    *c = fma(-sa, bs, fma(ca, cm, ca));
+}
+
 // 2^(j/64), j = 0 .. 63
 __constant__ static const double ss_exp_table[64] = {
     1.0, 1.0108892860517005, 1.0218971486541166, 1.0330248790212284,
