@@ -1334,7 +1334,7 @@ static __device__ __forceinline__ void ss_grad_dots(const ss_cd gam[SS_D], const
 // AE - EA with norm at most 2 nu |E|, so the
 // series left out (p + q > N = SS_ADJ_ORDER) is below
 // (2 nu)^(N+1) / ((N+1)! (N+2)) of |E|: 2.4e-12 at the sub-step bound
-// nu = 0.024, 6e-14 at the bench workload's 0.013.
+// nu = 0.024, 6e-14 at nu = 0.013.
 // An empty asm statement that reads and rewrites v: v is complete here, and
 // what uses it afterwards starts here (the statements keep their order).
 static __device__ __forceinline__ void ss_pin(ss_cd v[SS_D])
