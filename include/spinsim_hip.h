@@ -117,6 +117,13 @@ enum { /* layout of the batch's operator rows (no reference counterpart) */
 const char *ss_last_error(void);
 /* ABI version of this header (bumped on incompatible change). */
 int ss_abi_version(void);
+/* HIP events (hipEvent_t, created by the caller) that the next
+ * ss_time_evolution or ss_batch_time_evolution call on this thread stamps
+ * with its kernel's own start and end (hipExtModuleLaunchKernel): kernel
+ * timing without event markers between the kernels of a pass.  Either may be
+ * NULL; they apply to one launch and are then cleared.  Always returns 0.
+ * (No reference counterpart: measurement.) */
+int ss_time_evolution_events(void *start_event, void *stop_event);
 /* 64-bit hash of the embedded kernel sources (for code-object caches). */
 uint64_t ss_source_hash(void);
 

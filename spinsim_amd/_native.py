@@ -25,6 +25,7 @@ SIGNATURES = {
     "ss_last_error": (ctypes.c_char_p, []),
     "ss_abi_version": (c_int, []),
     "ss_source_hash": (ctypes.c_uint64, []),
+    "ss_time_evolution_events": (c_int, [c_vp, c_vp]),
     "ss_compile_code": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                 ctypes.POINTER(c_vp), ctypes.POINTER(ctypes.c_size_t),
                                 ctypes.POINTER(c_vp)]),

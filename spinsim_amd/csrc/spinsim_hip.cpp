@@ -4,6 +4,7 @@
 // are compiled per callback with hipRTC for gfx950 (the kernel text is
 // embedded in this library).  The state scan and Bloch-vector kernels do not,
 // and are compiled ahead of time into this library.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -1159,6 +1160,22 @@ std::string build_source(const char* field_src, int dim, int method, int expo, i
     return s;
 }
 
+// Events for the next time-evolution launch on this thread (ss_time_evolution_events).
+thread_local hipEvent_t g_te_start = nullptr, g_te_stop = nullptr;
+
+// Launch a time-evolution kernel of `blocks` workgroups of 256; with events
+// pending (ss_time_evolution_events) the launch itself stamps the kernel's
+// start and end into them -- no marker packets between the kernels of a
+// pass -- and clears them.
+hipError_t launch_te(hipFunction_t f, long long blocks, void** params, hipStream_t s)
+{
+    hipEvent_t a = g_te_start, b = g_te_stop;
+    g_te_start = g_te_stop = nullptr;
+    if (!a && !b) return hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    if (blocks * 256 > 4294967295LL) return hipErrorInvalidConfiguration;  // sizes in work-items here
+    return hipExtModuleLaunchKernel(f, uint32_t(blocks * 256), 1, 1, 256, 1, 1, 0, s, params, nullptr, a, b, 0);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1169,6 +1186,13 @@ extern "C" {
 const char* ss_last_error(void) { return g_error.c_str(); }
 
 int ss_abi_version(void) { return kAbiVersion; }
+
+int ss_time_evolution_events(void* start_event, void* stop_event)
+{
+    g_te_start = static_cast<hipEvent_t>(start_event);
+    g_te_stop = static_cast<hipEvent_t>(stop_event);
+    return 0;
+}
 
 uint64_t ss_source_hash(void)
 {
@@ -1420,8 +1444,7 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
     const long long blocks = (a.waves + 3) / 4;
     if (blocks > 2147483647LL) return fail("ss_time_evolution: grid too large");
     void* params[] = {&a};
-    SS_HIP(hipModuleLaunchKernel(lanes_per_step > 0 ? m->single : m->phased, unsigned(blocks), 1, 1, 256, 1, 1, 0,
-                                 static_cast<hipStream_t>(stream), params, nullptr));
+    SS_HIP(launch_te(lanes_per_step > 0 ? m->single : m->phased, blocks, params, static_cast<hipStream_t>(stream)));
     return 0;
 }
 
@@ -1482,7 +1505,7 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, s, a.counter);
     SS_HIP(hipGetLastError());
     void* params[] = {&a};
-    SS_HIP(hipModuleLaunchKernel(m->batch, unsigned(grid_blocks), 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    SS_HIP(launch_te(m->batch, grid_blocks, params, s));
     return 0;
 }
 
