@@ -121,7 +121,9 @@ int ss_abi_version(void);
  * ss_time_evolution or ss_batch_time_evolution call on this thread stamps
  * with its kernel's own start and end (hipExtModuleLaunchKernel): kernel
  * timing without event markers between the kernels of a pass.  Either may be
- * NULL; they apply to one launch and are then cleared.  Always returns 0.
+ * NULL.  They apply to the next such call only, which takes them on entry: a
+ * call that launches nothing (no rows, or an error) leaves them unrecorded.
+ * Always returns 0.
  * (No reference counterpart: measurement.) */
 int ss_time_evolution_events(void *start_event, void *stop_event);
 /* 64-bit hash of the embedded kernel sources (for code-object caches). */

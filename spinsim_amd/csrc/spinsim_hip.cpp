@@ -1160,20 +1160,31 @@ std::string build_source(const char* field_src, int dim, int method, int expo, i
     return s;
 }
 
-// Events for the next time-evolution launch on this thread (ss_time_evolution_events).
+// Events for the next time-evolution call on this thread (ss_time_evolution_events).
 thread_local hipEvent_t g_te_start = nullptr, g_te_stop = nullptr;
 
-// Launch a time-evolution kernel of `blocks` workgroups of 256; with events
-// pending (ss_time_evolution_events) the launch itself stamps the kernel's
-// start and end into them -- no marker packets between the kernels of a
-// pass -- and clears them.
-hipError_t launch_te(hipFunction_t f, long long blocks, void** params, hipStream_t s)
+struct TeEvents {
+    hipEvent_t start, stop;
+};
+
+// The pending events, cleared: each time-evolution call takes them on entry,
+// so they never outlive the call they were set for.
+TeEvents take_te_events()
 {
-    hipEvent_t a = g_te_start, b = g_te_stop;
+    const TeEvents e{g_te_start, g_te_stop};
     g_te_start = g_te_stop = nullptr;
-    if (!a && !b) return hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    return e;
+}
+
+// Launch a time-evolution kernel of `blocks` workgroups of 256; with events
+// the launch itself stamps the kernel's start and end into them (no marker
+// packets between the kernels of a pass).
+hipError_t launch_te(hipFunction_t f, long long blocks, void** params, hipStream_t s, TeEvents ev)
+{
+    if (!ev.start && !ev.stop) return hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, 256, 1, 1, 0, s, params, nullptr);
     if (blocks * 256 > 4294967295LL) return hipErrorInvalidConfiguration;  // sizes in work-items here
-    return hipExtModuleLaunchKernel(f, uint32_t(blocks * 256), 1, 1, 256, 1, 1, 0, s, params, nullptr, a, b, 0);
+    return hipExtModuleLaunchKernel(f, uint32_t(blocks * 256), 1, 1, 256, 1, 1, 0, s, params, nullptr, ev.start,
+                                    ev.stop, 0);
 }
 
 }  // namespace
@@ -1407,6 +1418,7 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
                       int64_t n_coarse, double* time_out, double* u_out, const ss_table* tables,
                       int lanes_per_step, ss_stream stream)
 {
+    const TeEvents ev = take_te_events();
     if (!m) return fail("ss_time_evolution: null module");
     if (n_coarse < 0) return fail("ss_time_evolution: negative n_coarse");
     if (n_coarse == 0) return 0;
@@ -1444,7 +1456,7 @@ int ss_time_evolution(const ss_module* m, const double* sweep, double t0, double
     const long long blocks = (a.waves + 3) / 4;
     if (blocks > 2147483647LL) return fail("ss_time_evolution: grid too large");
     void* params[] = {&a};
-    SS_HIP(launch_te(lanes_per_step > 0 ? m->single : m->phased, blocks, params, static_cast<hipStream_t>(stream)));
+    SS_HIP(launch_te(lanes_per_step > 0 ? m->single : m->phased, blocks, params, static_cast<hipStream_t>(stream), ev));
     return 0;
 }
 
@@ -1454,6 +1466,7 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
                             const ss_table* tables, uint64_t* counter, int lanes_per_step, int items_per_grab,
                             int grid_blocks, int u_layout, ss_stream stream)
 {
+    const TeEvents ev = take_te_events();
     if (!m) return fail("ss_batch_time_evolution: null module");
     if (u_layout != SS_U_FULL && !((u_layout == SS_U_SU2_COLUMN || u_layout == SS_U_SU2_FULL) && m->dim == 2))
         return fail("ss_batch_time_evolution: u_layout %d (SS_U_FULL, or SS_U_SU2_COLUMN / SS_U_SU2_FULL for "
@@ -1505,7 +1518,7 @@ int ss_batch_time_evolution(const ss_module* m, const double* sweep, int64_t swe
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, s, a.counter);
     SS_HIP(hipGetLastError());
     void* params[] = {&a};
-    SS_HIP(launch_te(m->batch, grid_blocks, params, s));
+    SS_HIP(launch_te(m->batch, grid_blocks, params, s, ev));
     return 0;
 }
 
