@@ -1229,6 +1229,14 @@ extern "C" __global__ void SS_KERNEL_ATTR ss_batch_time_evolution_kernel(ss_batc
 #define SS_ADJ_TAYLOR 7      // Taylor order of the exponentials walking psi and lambda back
 #define SS_ADJ_NU 0.024      // sub-step until the generator norm bound is below this
 #define SS_ADJ_MAX_SPLIT 11  // at most 2^11 sub-steps (norm bound up to 49)
+// Waves whose every lane's step has nu <= SS_ADJ_NU_SMALL (a 20 ns step of
+// controls of a few 2 pi 10 kHz in the rotating frame is ~1e-3) take one piece
+// with shorter series: Taylor order 5 (truncation below nu^6 / 6! = 5.7e-18
+// relative) and Frechet terms p + q <= 4 (below (2 nu)^5 / (5! 6) = 4.6e-14
+// of |E|).
+#define SS_ADJ_NU_SMALL 0.004
+#define SS_ADJ_TAYLOR_SMALL 5
+#define SS_ADJ_ORDER_SMALL 4
 
 struct ss_gen64 {
     double d[SS_D];
@@ -1318,7 +1326,7 @@ static __device__ __forceinline__ void ss_grad_dots(const ss_cd gam[SS_D], const
 }
 
 // One piece walked backward: psi <- e^{-A} psi, lambda <- e^{-A} lambda
-// (Taylor series to order SS_ADJ_TAYLOR: below nu^8 / 8! = 2.7e-18 relative
+// (Taylor series to order T; SS_ADJ_TAYLOR = 7: below nu^8 / 8! = 2.7e-18 relative
 // at the sub-step bound), and grad[k] += w Re(lambda^+ L(A, E_k) psi_before)
 // for the axes in mask, psi_before = e^{-A} psi.  With
 // L(A, E) psi_before = int_0^1 e^{sA} E e^{(1-s)A} psi_before ds
@@ -1332,7 +1340,7 @@ static __device__ __forceinline__ void ss_grad_dots(const ss_cd gam[SS_D], const
 // series streams it.  The terms
 // of order p + q = n sum to lambda^+ ad_A^n(E) psi / (n! (n+1)), ad_A(E) =
 // AE - EA with norm at most 2 nu |E|, so the
-// series left out (p + q > N = SS_ADJ_ORDER) is below
+// series left out (p + q > N; SS_ADJ_ORDER = 5) is below
 // (2 nu)^(N+1) / ((N+1)! (N+2)) of |E|: 2.4e-12 at the sub-step bound
 // nu = 0.024, 6e-14 at nu = 0.013.
 // An empty asm statement that reads and rewrites v: v is complete here, and
@@ -1343,36 +1351,37 @@ static __device__ __forceinline__ void ss_pin(ss_cd v[SS_D])
     for (int r = 0; r < SS_D; ++r) asm volatile("" : "+v"(v[r].x), "+v"(v[r].y));
 }
 
+template <int T, int N>
 static __device__ __forceinline__ void ss_adjoint_piece(const ss_gen64& na, ss_cd psi[SS_D], ss_cd lam[SS_D],
                                                         unsigned mask, double w, double grad[SS_LIE])
 {
     // lambda <- sum_p alpha_p / p!, alpha_p = na^p lambda streamed, each
     // alpha_p (p <= N) added into gamma_q for q <= N - p
-    ss_cd gam[SS_ADJ_ORDER + 1][SS_D];
+    ss_cd gam[N + 1][SS_D];
     {
         ss_cd al[SS_D], acc[SS_D];
 #pragma unroll
         for (int r = 0; r < SS_D; ++r) {
             al[r] = acc[r] = lam[r];
 #pragma unroll
-            for (int q = 0; q <= SS_ADJ_ORDER; ++q) gam[q][r] = ss_scale(lam[r], ss_inv_fact[q] / (q + 1));
+            for (int q = 0; q <= N; ++q) gam[q][r] = ss_scale(lam[r], ss_inv_fact[q] / (q + 1));
         }
 #pragma unroll
-        for (int p = 1; p <= SS_ADJ_TAYLOR; ++p) {
+        for (int p = 1; p <= T; ++p) {
             ss_cd t[SS_D];
             ss_gen_vec(na, al, t);
 #pragma unroll
             for (int r = 0; r < SS_D; ++r) al[r] = t[r];
             ss_axpy(acc, ss_inv_fact[p], al);
 #pragma unroll
-            for (int q = 0; q <= SS_ADJ_ORDER - p; ++q)
+            for (int q = 0; q <= N - p; ++q)
                 ss_axpy(gam[q], ss_inv_fact[p] * ss_inv_fact[q] / (p + q + 1), al);
         }
 #pragma unroll
         for (int r = 0; r < SS_D; ++r) lam[r] = acc[r];
     }
 #pragma unroll
-    for (int q = 0; q <= SS_ADJ_ORDER; ++q) ss_pin(gam[q]);
+    for (int q = 0; q <= N; ++q) ss_pin(gam[q]);
     // (left to itself the compiler interleaves the two streams, keeping the
     // powers alpha_p and the sums gamma_q live at once: 298 registers where
     // two waves per SIMD have 256, and the kernel spills)
@@ -1384,7 +1393,7 @@ static __device__ __forceinline__ void ss_adjoint_piece(const ss_gen64& na, ss_c
 #pragma unroll
         for (int r = 0; r < SS_D; ++r) bq[r] = acc[r] = psi[r];
 #pragma unroll
-        for (int q = 0; q <= SS_ADJ_TAYLOR; ++q) {
+        for (int q = 0; q <= T; ++q) {
             if (q > 0) {
                 ss_cd t[SS_D];
                 ss_gen_vec(na, bq, t);
@@ -1392,7 +1401,7 @@ static __device__ __forceinline__ void ss_adjoint_piece(const ss_gen64& na, ss_c
                 for (int r = 0; r < SS_D; ++r) bq[r] = t[r];
                 ss_axpy(acc, ss_inv_fact[q], bq);
             }
-            if (q <= SS_ADJ_ORDER) ss_grad_dots(gam[q], bq, mask, w, grad);
+            if (q <= N) ss_grad_dots(gam[q], bq, mask, w, grad);
         }
 #pragma unroll
         for (int r = 0; r < SS_D; ++r) psi[r] = acc[r];
@@ -1400,7 +1409,9 @@ static __device__ __forceinline__ void ss_adjoint_piece(const ss_gen64& na, ss_c
 }
 
 // One exponential walked backward: psi <- exp(-A) psi, lambda <- exp(-A) lambda,
-// grad += dL/dg.  Large steps are split into 2^s equal sub-steps.
+// grad += dL/dg.  Large steps are split into 2^s equal sub-steps; a wave whose
+// steps are all small takes the shorter series (the choice is per wave, so
+// its lanes stay on one path).
 static __device__ __forceinline__ void ss_adjoint_exp(const double g[SS_FIELD_LEN], ss_cd psi[SS_D], ss_cd lam[SS_D],
                                                       unsigned mask, double grad[SS_LIE], int* status)
 {
@@ -1414,9 +1425,15 @@ static __device__ __forceinline__ void ss_adjoint_exp(const double g[SS_FIELD_LE
     if (nu * ldexp(1.0, -split) > SS_ADJ_NU) atomicOr(status, 1);
     const double scale = ldexp(1.0, -split);
     const ss_gen64 na = ss_generator64(g, -scale);  // -A
+    if (__all(nu <= SS_ADJ_NU_SMALL)) {
+        // every lane of the wave below the small bound: one piece, shorter series
+        ss_adjoint_piece<SS_ADJ_TAYLOR_SMALL, SS_ADJ_ORDER_SMALL>(na, psi, lam, mask, 1.0, grad);
+        return;
+    }
     const int pieces = 1 << split;
 #pragma unroll 1
-    for (int r = 0; r < pieces; ++r) ss_adjoint_piece(na, psi, lam, mask, scale, grad);  // d/dg = scale * d/dg_s
+    for (int r = 0; r < pieces; ++r)
+        ss_adjoint_piece<SS_ADJ_TAYLOR, SS_ADJ_ORDER>(na, psi, lam, mask, scale, grad);  // d/dg = scale * d/dg_s
 }
 
 // Cubic Hermite interval i of a table: value = a_i h00 + a_{i+1} h01 + h L h10
